@@ -1,0 +1,295 @@
+// Cell binning and pair-list kernels.
+//
+// Binning replaces kernel_cellmap! + kernel_fillcells_naive!
+// (src/gpukernels_cls.jl:29-101) by a counting sort: a cell histogram with an
+// arrival slot per particle, an exclusive scan into cell start offsets, an
+// unordered scatter, and a per-cell ranking by input index that makes the order
+// deterministic.  The pair search replaces kernel_neib_internal!/external!
+// (:309-554): every sorted particle scans the 3^D cells around it (the rows of a
+// stencil are contiguous in cell-sorted order) and keeps partners with a larger
+// sorted index, so each unordered pair is produced once.
+#include "celllist.hpp"
+
+namespace gclsph {
+namespace {
+
+template <typename T>
+struct PrecTraits;
+template <>
+struct PrecTraits<double> {
+  static __host__ __device__ double off(const CellGeom& g, int k) { return g.off64[k]; }
+  static __host__ __device__ double hinv(const CellGeom& g, int k) { return g.hinv64[k]; }
+};
+template <>
+struct PrecTraits<float> {
+  static __host__ __device__ float off(const CellGeom& g, int k) { return g.off32[k]; }
+  static __host__ __device__ float hinv(const CellGeom& g, int k) { return g.hinv32[k]; }
+};
+
+// r^2 as the reference writes it, (a1-b1)^2 + (a2-b2)^2 [+ (a3-b3)^2], each
+// operation rounded on its own (src/gpukernels_cls.jl:326,390,456,527).
+template <typename T, int D>
+__device__ __forceinline__ T dist2_exact(const T* const* x, int64_t a, int64_t b) {
+#pragma clang fp contract(off)
+  T d0 = x[0][a] - x[0][b];
+  T d1 = x[1][a] - x[1][b];
+  T s = d0 * d0 + d1 * d1;
+  if (D == 3) {
+    T d2 = x[2][a] - x[2][b];
+    s = s + d2 * d2;
+  }
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ int32_t cell_coord(T xv, const CellGeom& g, int k, int use_off64) {
+#pragma clang fp contract(off)
+  double p;
+  if (use_off64) {
+    p = (double(xv) - g.off64[k]) * double(PrecTraits<T>::hinv(g, k));
+  } else {
+    T pt = (xv - PrecTraits<T>::off(g, k)) * PrecTraits<T>::hinv(g, k);
+    p = double(pt);
+  }
+  // min(max(p, 1), grid); a NaN coordinate lands in cell 1 so every index
+  // stays inside the grid (the reference leaves it undefined)
+  double cl = p >= 1.0 ? p : 1.0;
+  double gk = double(g.grid[k]);
+  cl = cl <= gk ? cl : gk;
+  return int32_t(ceil(cl));
+}
+
+struct XPtrs {
+  const void* p[3];
+};
+
+template <typename T, int D>
+__global__ void k_cellmap(XPtrs xp, XPtrs xalt, const int* sel, int64_t n, CellGeom g, int use_off64,
+                          int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ cnt,
+                          const int* flag) {
+  if (flag && *flag == 0) return;
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const XPtrs& src = (sel && *sel) ? xalt : xp;
+  int32_t c[3] = {1, 1, 1};
+#pragma unroll
+  for (int k = 0; k < D; ++k) c[k] = cell_coord<T>(static_cast<const T*>(src.p[k])[i], g, k, use_off64);
+  int64_t lin = int64_t(c[0] - 1) + g.grid[0] * (int64_t(c[1] - 1) + g.grid[1] * int64_t(c[2] - 1));
+  key[i] = int32_t(lin);
+  slot[i] = atomicAdd(&cnt[lin], 1);
+}
+
+__global__ void k_scatter(const int32_t* __restrict__ key, const int32_t* __restrict__ slot,
+                          const int32_t* __restrict__ start, int32_t* __restrict__ tmp, int64_t n, const int* flag) {
+  if (flag && *flag == 0) return;
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  tmp[start[key[i]] + slot[i]] = int32_t(i);
+}
+
+// Rank each item inside its cell by input index: final order is (cell, input).
+__global__ void k_rank(const int32_t* __restrict__ key, const int32_t* __restrict__ start,
+                       const int32_t* __restrict__ tmp, int32_t* __restrict__ perm, int32_t* __restrict__ skey,
+                       int64_t n, const int* flag) {
+  if (flag && *flag == 0) return;
+  int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int32_t i = tmp[s];
+  int32_t c = key[i];
+  int32_t b = start[c], e = start[c + 1];
+  int32_t r = 0;
+  for (int32_t k = b; k < e; ++k) r += tmp[k] < i;
+  perm[b + r] = i;
+  skey[b + r] = c;
+}
+
+template <typename T, int D>
+__global__ void k_gather_pos(XPtrs xp, XPtrs xsp, const int32_t* __restrict__ perm, int64_t n) {
+  int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int32_t i = perm[s];
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    const_cast<T*>(static_cast<const T*>(xsp.p[k]))[s] = static_cast<const T*>(xp.p[k])[i];
+}
+
+template <typename T, int D>
+__global__ void k_pair_count(XPtrs xsp, const int32_t* __restrict__ start, const int32_t* __restrict__ skey,
+                             int64_t n, CellGeom g, T dist2max, int32_t* __restrict__ cnt) {
+  int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
+                   static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
+  Rows r = stencil_rows<D>(g, start, skey[s]);
+  int32_t c = 0;
+  for (int k = 0; k < r.n; ++k) {
+    int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
+    for (int32_t t = b; t < r.e[k]; ++t) c += dist2_exact<T, D>(x, s, t) < dist2max;
+  }
+  cnt[s] = c;
+}
+
+template <typename T, int D>
+__global__ void k_pair_emit(XPtrs xsp, const int32_t* __restrict__ start, const int32_t* __restrict__ skey,
+                            const int32_t* __restrict__ id, int64_t n, CellGeom g, T dist2max,
+                            const int64_t* __restrict__ off, int32_t* __restrict__ pairs, int64_t capacity) {
+  int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int64_t o = off[s];
+  if (o >= capacity) return;
+  const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
+                   static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
+  Rows r = stencil_rows<D>(g, start, skey[s]);
+  int32_t is = id[s] + 1;
+  for (int k = 0; k < r.n; ++k) {
+    int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
+    for (int32_t t = b; t < r.e[k]; ++t) {
+      if (dist2_exact<T, D>(x, s, t) < dist2max) {
+        if (o < capacity) {
+          int32_t it = id[t] + 1;
+          int2 pr = is < it ? make_int2(is, it) : make_int2(it, is);
+          reinterpret_cast<int2*>(pairs)[o] = pr;
+        }
+        ++o;
+      }
+    }
+  }
+}
+
+template <typename T>
+void bin_sort_t(const CellGeom& g, const void* const* x, int64_t n, bool use_off64, SortBuffers& sb, const int* flag,
+                hipStream_t s, const void* const* x_alt, const int* sel) {
+  XPtrs xp{{x[0], x[1], g.dim == 3 ? x[2] : x[1]}};
+  XPtrs xa = xp;
+  if (x_alt) xa = XPtrs{{x_alt[0], x_alt[1], g.dim == 3 ? x_alt[2] : x_alt[1]}};
+  int64_t nc = g.ncells();
+  fill_i32(sb.cnt, nc + 1, 0, flag, s);
+  if (g.dim == 2)
+    k_cellmap<T, 2><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag);
+  else
+    k_cellmap<T, 3><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag);
+  GCL_HIP(hipGetLastError());
+  fill_i32(sb.maxcell, 1, 0, flag, s);
+  exclusive_scan_i32(sb.cnt, sb.start, nc, sb.scan_tmp, flag, s, sb.maxcell);
+  k_scatter<<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.slot, sb.start, sb.tmp, n, flag);
+  GCL_HIP(hipGetLastError());
+  k_rank<<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.start, sb.tmp, sb.perm, sb.skey, n, flag);
+  GCL_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void SortBuffers::alloc(int64_t n_, int64_t ncells_) {
+  release();
+  n = n_;
+  ncells = ncells_;
+  GCL_HIP(hipMalloc(&key, sizeof(int32_t) * size_t(n)));
+  GCL_HIP(hipMalloc(&slot, sizeof(int32_t) * size_t(n)));
+  GCL_HIP(hipMalloc(&tmp, sizeof(int32_t) * size_t(n)));
+  GCL_HIP(hipMalloc(&perm, sizeof(int32_t) * size_t(n)));
+  GCL_HIP(hipMalloc(&skey, sizeof(int32_t) * size_t(n)));
+  GCL_HIP(hipMalloc(&cnt, sizeof(int32_t) * size_t(ncells + 1)));
+  GCL_HIP(hipMalloc(&start, sizeof(int32_t) * size_t(ncells + 1)));
+  GCL_HIP(hipMalloc(&scan_tmp, sizeof(int64_t) * scan_tmp_elems(ncells)));
+  GCL_HIP(hipMalloc(&maxcell, sizeof(int32_t) * 4));
+}
+
+void SortBuffers::release() {
+  for (void* p : {(void*)key, (void*)slot, (void*)tmp, (void*)perm, (void*)skey, (void*)cnt, (void*)start,
+                  (void*)scan_tmp, (void*)maxcell})
+    if (p) (void)hipFree(p);
+  key = slot = tmp = perm = skey = cnt = start = maxcell = nullptr;
+  scan_tmp = nullptr;
+}
+
+void PairBuffers::alloc(int64_t n_) {
+  release();
+  n = n_;
+  GCL_HIP(hipMalloc(&cnt, sizeof(int32_t) * size_t(n + 1)));
+  GCL_HIP(hipMalloc(&off, sizeof(int64_t) * size_t(n + 1)));
+  GCL_HIP(hipMalloc(&scan_tmp, sizeof(int64_t) * scan_tmp_elems(n)));
+}
+
+void PairBuffers::set_capacity(int64_t cap, hipStream_t s) {
+  if (pairs) GCL_HIP(hipFree(pairs));
+  pairs = nullptr;
+  capacity = cap;
+  if (cap > 0) {
+    GCL_HIP(hipMalloc(&pairs, sizeof(int32_t) * 2 * size_t(cap)));
+    GCL_HIP(hipMemsetAsync(pairs, 0, sizeof(int32_t) * 2 * size_t(cap), s));
+  }
+}
+
+void PairBuffers::release() {
+  for (void* p : {(void*)cnt, (void*)off, (void*)scan_tmp, (void*)pairs})
+    if (p) (void)hipFree(p);
+  cnt = nullptr;
+  off = nullptr;
+  scan_tmp = nullptr;
+  pairs = nullptr;
+  capacity = 0;
+}
+
+void bin_sort(const CellGeom& g, int prec, const void* const* x, int64_t n, bool use_off64, SortBuffers& sb,
+              const int* flag, hipStream_t s, const void* const* x_alt, const int* sel) {
+  if (prec == 32) bin_sort_t<float>(g, x, n, use_off64, sb, flag, s, x_alt, sel);
+  else bin_sort_t<double>(g, x, n, use_off64, sb, flag, s, x_alt, sel);
+}
+
+void gather_positions(int dim, int prec, const void* const* x, void* const* xs, const int32_t* perm, int64_t n,
+                      hipStream_t s) {
+  XPtrs xp{{x[0], x[1], dim == 3 ? x[2] : x[1]}};
+  XPtrs xsp{{xs[0], xs[1], dim == 3 ? xs[2] : xs[1]}};
+  if (prec == 32) {
+    if (dim == 2) k_gather_pos<float, 2><<<grid_for(n), kBlock, 0, s>>>(xp, xsp, perm, n);
+    else k_gather_pos<float, 3><<<grid_for(n), kBlock, 0, s>>>(xp, xsp, perm, n);
+  } else {
+    if (dim == 2) k_gather_pos<double, 2><<<grid_for(n), kBlock, 0, s>>>(xp, xsp, perm, n);
+    else k_gather_pos<double, 3><<<grid_for(n), kBlock, 0, s>>>(xp, xsp, perm, n);
+  }
+  GCL_HIP(hipGetLastError());
+}
+
+void pair_count(const CellGeom& g, int prec, const void* const* xs, const int32_t* start, const int32_t* skey,
+                int64_t n, double dist, PairBuffers& pb, hipStream_t s) {
+  XPtrs xsp{{xs[0], xs[1], g.dim == 3 ? xs[2] : xs[1]}};
+  if (prec == 32) {
+    float d = float(dist);
+    float d2 = d * d;
+    if (g.dim == 2) k_pair_count<float, 2><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
+    else k_pair_count<float, 3><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
+  } else {
+    double d2 = dist * dist;
+    if (g.dim == 2) k_pair_count<double, 2><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
+    else k_pair_count<double, 3><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
+  }
+  GCL_HIP(hipGetLastError());
+  exclusive_scan_i64(pb.cnt, pb.off, n, pb.scan_tmp, nullptr, s);
+}
+
+void pair_emit(const CellGeom& g, int prec, const void* const* xs, const int32_t* start, const int32_t* skey,
+               const int32_t* id, int64_t n, double dist, PairBuffers& pb, hipStream_t s) {
+  if (pb.capacity <= 0) return;
+  XPtrs xsp{{xs[0], xs[1], g.dim == 3 ? xs[2] : xs[1]}};
+  if (prec == 32) {
+    float d = float(dist);
+    float d2 = d * d;
+    if (g.dim == 2)
+      k_pair_emit<float, 2><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
+                                                           pb.capacity);
+    else
+      k_pair_emit<float, 3><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
+                                                           pb.capacity);
+  } else {
+    double d2 = dist * dist;
+    if (g.dim == 2)
+      k_pair_emit<double, 2><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
+                                                            pb.capacity);
+    else
+      k_pair_emit<double, 3><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
+                                                            pb.capacity);
+  }
+  GCL_HIP(hipGetLastError());
+}
+
+}  // namespace gclsph

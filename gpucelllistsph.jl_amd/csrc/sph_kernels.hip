@@ -1,0 +1,1044 @@
+// Particle-centric SPH step kernels.
+//
+// Every pair kernel of the reference (src/gpukernels_sph.jl) runs one thread per
+// *pair* and scatters both halves with FP64 atomics.  Here one thread owns one
+// particle `a` of the cell-sorted state and gathers over the 3^D cells around
+// it; the pairs it visits are exactly the reference's neighbour list (same
+// cells, r^2 < dist^2 at the positions of the last rebuild).  For the pair
+// (a, b) it evaluates the reference's per-pair expression for a's side — with
+// Δx = x_a - x_b, which is the reference's Δx or its exact negation — so the
+// per-pair values match the reference's and only the summation order differs.
+// Sums are per-thread and deterministic; there are no floating-point atomics.
+//
+// Pass 1 (tⁿ) fuses sphW!/sph∇W!, ∂ρ∂tDDT!, pressure!, ∂v∂t!, ∂v∂t_av!,
+// ∂v∂t_visc!, ∂v∂t_addgrav!, fbmolforce!, ∂v∂tpF! and the three half-step
+// updates (src/sphproblem.jl:244-274).  Pass 2 (t^{n+1/2}) fuses the second
+// evaluation with symplectic_update! and the Δt / rebuild reductions
+// (src/sphproblem.jl:277-335).
+//
+// R = double: the Float64 path.  Every expression is evaluated in the
+// reference's operation order with FMA contraction off, so branch decisions
+// (r < r0, cond < 0, clamps) see the same values as the reference would.
+// R = float: the FP32-split path (density stored as ρ-ρ0, Kahan pair sums,
+// Float64 positions and Δt).
+#include <type_traits>
+
+#include "sph.hpp"
+
+namespace gclsph {
+
+// ---------------------------------------------------------------- math helpers
+__device__ __forceinline__ double powi_d(double x, int p) {
+#pragma clang fp contract(off)
+  // binary powering, same operation sequence as the oracle's powi
+  double r = 1.0, b = x;
+  bool first = true;
+  while (p > 0) {
+    if (p & 1) {
+      r = first ? b : r * b;
+      first = false;
+    }
+    p >>= 1;
+    if (p) b = b * b;
+  }
+  return r;
+}
+
+// powfancy7th (src/gpukernels_sph.jl:499-515) for γ == 7; x^(1/γ) otherwise.
+__device__ __forceinline__ double pf7(double x, int gamma, double ginv) {
+#pragma clang fp contract(off)
+  if (gamma == 7) {
+    unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(x));
+    unsigned long long tb = 0x36cd000000000000ull + bits / 7ull;
+    double t = copysign(__longlong_as_double((long long)tb), x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      double t2 = t * t;
+      double t3 = t2 * t;
+      double t4 = t2 * t2;
+      double xot4 = x / t4;
+      t = t - t * (t3 - xot4) / (4.0 * t3 + 3.0 * xot4);
+    }
+    return t;
+  }
+  return pow(x, ginv);
+}
+
+// WendlandC2 (SPHKernels.jl): W = ((1+4u)(1-u)^4) σ H⁻ᴰ, dW = -20u(1-u)^3 σ H⁻⁽ᴰ⁺¹⁾.
+__device__ __forceinline__ double W_d(double u, double wnorm) {
+#pragma clang fp contract(off)
+  if (u < 1.0) {
+    double um1 = 1.0 - u;
+    um1 = um1 * um1;
+    um1 = um1 * um1;
+    return ((1.0 + 4.0 * u) * um1) * wnorm;
+  }
+  return 0.0;
+}
+__device__ __forceinline__ double dW_d(double u, double dwnorm) {
+#pragma clang fp contract(off)
+  if (u < 1.0) {
+    double um1 = 1.0 - u;
+    double um13 = um1 * um1 * um1;
+    return -20.0 * u * um13 * dwnorm;
+  }
+  return 0.0;
+}
+__device__ __forceinline__ float W_f(float u, float wnorm) {
+  if (u < 1.0f) {
+    float um1 = 1.0f - u;
+    um1 = um1 * um1;
+    um1 = um1 * um1;
+    return ((1.0f + 4.0f * u) * um1) * wnorm;
+  }
+  return 0.0f;
+}
+__device__ __forceinline__ float dW_f(float u, float dwnorm) {
+  if (u < 1.0f) {
+    float um1 = 1.0f - u;
+    return -20.0f * u * (um1 * um1 * um1) * dwnorm;
+  }
+  return 0.0f;
+}
+
+// Tait EOS (src/gpukernels_sph.jl:691-697): 0 if ptype <= 1 and ρ < ρ0.
+__device__ __forceinline__ double eos_d(double rho, int32_t pt, const SphConst& C) {
+#pragma clang fp contract(off)
+  if (pt <= 1 && rho < C.rho0) return 0.0;
+  return C.P0 * (powi_d(rho / C.rho0, C.gamma) - 1);
+}
+// FP32-split EOS on δρ: P0((1+e)^7 - 1) = P0 e (7 + 21e + 35e² + 35e³ + 21e⁴ + 7e⁵ + e⁶).
+__device__ __forceinline__ float eos_f(float drho, int32_t pt, const SphConst& C) {
+  if (pt <= 1 && drho < 0.0f) return 0.0f;
+  float e = drho / C.f_rho0;
+  float poly = 7.0f + e * (21.0f + e * (35.0f + e * (35.0f + e * (21.0f + e * (7.0f + e)))));
+  return C.f_P0 * (e * poly);
+}
+// ρ0((1+z)^(1/7) - 1) without cancellation (FP32-split form of
+// ρ0*powfancy7th(1+z) - ρ0): binomial series, exact function for |z| > 1e-2.
+__device__ __forceinline__ float hydro_drho_f(float z, float rho0) {
+  if (fabsf(z) < 1e-2f) {
+    float t = z * (1.0f / 7.0f + z * (-3.0f / 49.0f + z * (13.0f / 343.0f + z * (-65.0f / 2401.0f))));
+    return rho0 * t;
+  }
+  return rho0 * expm1f(log1pf(z) * (1.0f / 7.0f));
+}
+
+template <typename R>
+struct Sum {  // plain sum
+  R s = R(0);
+  __device__ __forceinline__ void add(R v) { s += v; }
+  __device__ __forceinline__ R get() const { return s; }
+};
+template <>
+struct Sum<float> {  // Kahan-compensated sum
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+  __device__ __forceinline__ float get() const { return s; }
+};
+
+template <int D>
+__device__ __forceinline__ bool member(const double* const* XM, int64_t a, int64_t b, double dist2) {
+#pragma clang fp contract(off)
+  double d0 = XM[0][a] - XM[0][b];
+  double d1 = XM[1][a] - XM[1][b];
+  double s = d0 * d0 + d1 * d1;
+  if (D == 3) {
+    double d2 = XM[2][a] - XM[2][b];
+    s = s + d2 * d2;
+  }
+  return s < dist2;
+}
+
+template <int D>
+__device__ __forceinline__ double dotd(const double* a, const double* b) {
+#pragma clang fp contract(off)
+  double s = a[0] * b[0] + a[1] * b[1];
+  if (D == 3) s = s + a[2] * b[2];
+  return s;
+}
+template <int D>
+__device__ __forceinline__ float dotf(const float* a, const float* b) {
+  float s = a[0] * b[0] + a[1] * b[1];
+  if (D == 3) s = s + a[2] * b[2];
+  return s;
+}
+
+// Positions of the pair-list membership test: the current positions when this
+// step rebuilt (or when no later step can skip a rebuild), else the stored
+// rebuild positions.
+template <int D, typename R>
+__device__ __forceinline__ void membership_ptrs(const SphBufs<R>& B, const StepFlags& F, int p, int rebuilt,
+                                                const double** XM) {
+  bool cur = rebuilt || !F.need_xrb;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) XM[k] = cur ? B.X[p][k < D ? k : 1] : B.XRB[k < D ? k : 1];
+}
+
+__device__ __forceinline__ void reduce_max(unsigned long long* slot, double v, bool active) {
+  double m = wave_max(active ? v : 0.0);
+  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(slot, m);
+}
+
+// ------------------------------------------------------------------ kernels
+// Runs after the (conditional) rebuild kernels: commits the reorder.
+__global__ void k_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (ctl->rebuild) {
+    ctl->parity ^= 1;
+    ctl->pparity ^= 1;
+    ctl->rebuilt = 1;
+    ctl->updaten += 1;
+    long long mp = *maxcell;
+    if (mp > ctl->mppcell) ctl->mppcell = (long long)ceil(double(mp) * 1.05 + 1);  // src/celllist.jl:134-135
+    ctl->rebuild = 0;
+    ctl->xrb_cur = write_xrb ? 0 : 1;
+  } else {
+    ctl->rebuilt = 0;
+  }
+}
+
+// New sorted position s takes the state of old position perm[s].
+template <int D, typename R>
+__global__ void k_reorder(SphBufs<R> B, StepFlags F) {
+  const StepCtl* ctl = B.ctl;
+  if (ctl->rebuild == 0) return;
+  int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= B.n) return;
+  const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, pq = pp ^ 1;
+  const int32_t src = B.perm[s];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    double xv = B.X[p][k][src];
+    B.X[q][k][s] = xv;
+    B.V[q][k][s] = B.V[p][k][src];
+    if (F.write_xrb) B.XRB[k][s] = xv;
+    if (!F.synced_prev) {  // previous half-step state moves with its particle
+      B.XH[k][s] = B.XH2[k][src];
+      B.VH[k][s] = B.VH2[k][src];
+    }
+  }
+  B.RHO[q][s] = B.RHO[p][src];
+  if (!F.synced_prev) B.RH[s] = B.RH2[src];
+  B.PT[pq][s] = B.PT[pp][src];
+  B.ID[pq][s] = B.ID[pp][src];
+}
+
+// ---- pass 1 ---------------------------------------------------------------
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+#pragma clang fp contract(off)
+  const StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, pp = ctl->pparity, rebuilt = ctl->rebuilt;
+  const double dt = ctl->dt;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= B.n) return;
+  const double* X[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
+  const R* V[3] = {B.V[p][0], B.V[p][1], B.V[p][D == 3 ? 2 : 1]};
+  const R* RHO = B.RHO[p];
+  const int32_t* PT = B.PT[pp];
+  const double* XM[3];
+  membership_ptrs<D>(B, F, p, rebuilt, XM);
+
+  double xa[3];
+  R va[3];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    xa[k] = X[k][a];
+    va[k] = V[k][a];
+  }
+  const R sa = RHO[a];
+  const int32_t pta = PT[a];
+  Sum<R> acc[3], drho;
+
+  const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  if constexpr (std::is_same<R, double>::value) {
+    const double rhoa = sa;
+    const double Pa = eos_d(rhoa, pta, C);
+    for (int k = 0; k < rows.n; ++k) {
+      for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
+        if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
+        const int32_t ptb = PT[b];
+        double dx[3], dv[3], gw[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          dx[d] = xa[d] - X[d][b];
+          dv[d] = va[d] - V[d][b];
+        }
+        const double r2 = dotd<D>(dx, dx);
+        const double r = sqrt(r2);
+        const double u = r * C.Hinv;
+        const double dwk_r = dW_d(u, C.dwnorm) / r;
+#pragma unroll
+        for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
+        const double rhob = RHO[b];
+        const bool both = pta > 0 && ptb > 0;
+        double ax[3] = {0.0, 0.0, 0.0};
+        if (both) {  // ∂ρ∂tDDT! (:579-625)
+          double dot3 = -dotd<D>(dx, gw);
+          double s = C.m0 * dotd<D>(dv, gw);
+          if (pta > 1) {
+            double drhop = C.rho0 * pf7(1 + C.DDTgz * dx[1], C.gamma, C.ginv) - C.rho0;
+            double vd = C.DDTkh * (rhob - rhoa - drhop) / (r2 + C.eta2);
+            s += vd * dot3 * C.m0 / rhob;
+          }
+          drho.add(s);
+        }
+        if ((long long)pta * (long long)ptb > 0) {  // ∂v∂t! (:755-772)
+          double Pfac = (Pa + eos_d(rhob, ptb, C)) / (rhoa * rhob);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
+        }
+        if (both) {  // ∂v∂t_av! (:869-897)
+          double cond = dotd<D>(dv, dx);
+          if (cond < 0) {
+            double dmu = C.h * cond / (r2 + C.eta2);
+            double dPi = (-C.alpha * C.c0 * dmu) / ((rhoa + rhob) * 0.5);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += -dPi * C.m0 * gw[d];
+          }
+          if (F.nu_on) {  // ∂v∂t_visc! (:967-980)
+            double term = 4 * C.nu * C.m0 * dotd<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.eta2));
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+          }
+          if (F.s_on && r < C.st_2h) {  // ∂v∂tpF! (:1387-1397)
+            double scos = C.s * cos(C.st_c * r / C.st_2h) / (r + C.st_01h);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.m0;
+          }
+        }
+        if ((pta > 0) != (ptb > 0) && r < C.fb_r0[0]) {  // fbmolforce! (:1891-1901)
+          double q1 = C.fb_r0[0] / (r + C.fb_eta2[0]);
+          double Fc = C.fb_d[0] * (powi_d(q1, 12) - powi_d(q1, 4)) / (r2 + C.fb_eta2[0]);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += dx[d] * Fc;
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+      }
+    }
+    // ∂v∂t_addgrav!, then the half-step updates (:270-274)
+    double av[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) av[d] = acc[d].get();
+    av[1] = av[1] - C.g;
+    double rh0, vh0[3], xh0[3];
+    if (F.synced_prev) {
+      rh0 = rhoa;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        vh0[d] = va[d];
+        xh0[d] = xa[d];
+      }
+    } else {
+      const bool mv = rebuilt;  // the reorder moved the old half-step state to XH/VH/RH
+      rh0 = mv ? B.RH[a] : B.RH2[a];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        vh0[d] = mv ? B.VH[d][a] : B.VH2[d][a];
+        xh0[d] = mv ? B.XH[d][a] : B.XH2[d][a];
+      }
+    }
+    const double hdt = dt * 0.5;
+    double rv = rh0 + drho.get() * hdt;
+    if (rv < C.rho0 && pta < 0) rv = C.rho0;
+    B.RH[a] = rv;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double vv = vh0[d];
+      if (pta > 1) vv += av[d] * hdt;
+      B.VH[d][a] = vv;
+      B.XH[d][a] = xh0[d] + vv * hdt;
+    }
+  } else {
+    // ---------------- FP32-split
+#pragma clang fp contract(fast)
+    const float dra = sa;
+    const float rhoa = C.f_rho0 + dra;
+    const float Pa = eos_f(dra, pta, C);
+    for (int k = 0; k < rows.n; ++k) {
+      for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
+        if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
+        const int32_t ptb = PT[b];
+        float dx[3], dv[3], gw[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          dx[d] = float(xa[d] - X[d][b]);
+          dv[d] = va[d] - V[d][b];
+        }
+        const float r2 = dotf<D>(dx, dx);
+        const float r = sqrtf(r2);
+        const float u = r * C.f_Hinv;
+        const float dwk_r = dW_f(u, C.f_dwnorm) / r;
+#pragma unroll
+        for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
+        const float drb = RHO[b];
+        const float rhob = C.f_rho0 + drb;
+        const bool both = pta > 0 && ptb > 0;
+        float ax[3] = {0.0f, 0.0f, 0.0f};
+        if (both) {
+          float dot3 = -dotf<D>(dx, gw);
+          float s = C.f_m0 * dotf<D>(dv, gw);
+          if (pta > 1) {
+            float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
+            float vd = C.f_DDTkh * ((drb - dra) - drhop) / (r2 + C.f_eta2);
+            s += vd * dot3 * C.f_m0 / rhob;
+          }
+          drho.add(s);
+        }
+        if ((long long)pta * (long long)ptb > 0) {
+          float Pfac = (Pa + eos_f(drb, ptb, C)) / (rhoa * rhob);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
+        }
+        if (both) {
+          float cond = dotf<D>(dv, dx);
+          if (cond < 0.0f) {
+            float dmu = C.f_h * cond / (r2 + C.f_eta2);
+            float dPi = (-C.f_alpha * C.f_c0 * dmu) / ((rhoa + rhob) * 0.5f);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
+          }
+          if (F.nu_on) {
+            float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.f_eta2));
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+          }
+          if (F.s_on && r < C.f_st_2h) {
+            float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.f_m0;
+          }
+        }
+        if ((pta > 0) != (ptb > 0) && r < C.f_fb_r0[0]) {
+          float q1 = C.f_fb_r0[0] / (r + C.f_fb_eta2[0]);
+          float q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
+          float Fc = C.f_fb_d[0] * (q4 * q8 - q4) / (r2 + C.f_fb_eta2[0]);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += dx[d] * Fc;
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+      }
+    }
+    float av[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) av[d] = acc[d].get();
+    av[1] = av[1] - C.f_g;
+    float rh0, vh0[3];
+    double xh0[3];
+    if (F.synced_prev) {
+      rh0 = dra;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        vh0[d] = va[d];
+        xh0[d] = xa[d];
+      }
+    } else {
+      const bool mv = rebuilt;
+      rh0 = mv ? B.RH[a] : B.RH2[a];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        vh0[d] = mv ? B.VH[d][a] : B.VH2[d][a];
+        xh0[d] = mv ? B.XH[d][a] : B.XH2[d][a];
+      }
+    }
+    const float hdt = float(dt * 0.5);
+    float rv = rh0 + drho.get() * hdt;
+    if (rv < 0.0f && pta < 0) rv = 0.0f;
+    B.RH[a] = rv;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float vv = vh0[d];
+      if (pta > 1) vv += av[d] * hdt;
+      B.VH[d][a] = vv;
+      B.XH[d][a] = xh0[d] + double(vv) * (dt * 0.5);
+    }
+  }
+}
+
+// ---- pass 2 + symplectic update ------------------------------------------
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+#pragma clang fp contract(off)
+  StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
+  const double dt = ctl->dt;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = a < B.n;
+  double red_cdx = 0.0, red_acc = 0.0, red_visc = 0.0;
+  if (active) {
+    const double* X[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
+    const double* XH[3] = {B.XH[0], B.XH[1], B.XH[D == 3 ? 2 : 1]};
+    const R* VH[3] = {B.VH[0], B.VH[1], B.VH[D == 3 ? 2 : 1]};
+    const R* RH = B.RH;
+    const int32_t* PT = B.PT[pp];
+    const double* XM[3];
+    membership_ptrs<D>(B, F, p, rebuilt, XM);
+    double xa[3], xha[3];
+    R vha[3];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      xa[k] = X[k][a];
+      xha[k] = XH[k][a];
+      vha[k] = VH[k][a];
+    }
+    const R sha = RH[a];
+    const int32_t pta = PT[a];
+    Sum<R> acc[3], drho;
+    const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
+    R av[3];
+    R Pha, drho_v;
+    if constexpr (std::is_same<R, double>::value) {
+      const double rhoa = sha;
+      Pha = eos_d(rhoa, pta, C);
+      for (int k = 0; k < rows.n; ++k) {
+        for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
+          if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
+          const int32_t ptb = PT[b];
+          double dxn[3], dx[3], dv[3], gw[3];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            dxn[d] = xa[d] - X[d][b];
+            dx[d] = xha[d] - XH[d][b];
+            dv[d] = vha[d] - VH[d][b];
+          }
+          // ∇W from pass 1 (positions xⁿ, src/sphproblem.jl:246,281)
+          const double r2n = dotd<D>(dxn, dxn);
+          const double rn = sqrt(r2n);
+          const double dwk_r = dW_d(rn * C.Hinv, C.dwnorm) / rn;
+#pragma unroll
+          for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
+          const double r2 = dotd<D>(dx, dx);
+          const double rhob = RH[b];
+          const bool both = pta > 0 && ptb > 0;
+          double ax[3] = {0.0, 0.0, 0.0};
+          if (both) {
+            double dot3 = -dotd<D>(dx, gw);
+            double s = C.m0 * dotd<D>(dv, gw);
+            if (pta > 1) {
+              double drhop = C.rho0 * pf7(1 + C.DDTgz * dx[1], C.gamma, C.ginv) - C.rho0;
+              double vd = C.DDTkh * (rhob - rhoa - drhop) / (r2 + C.eta2);
+              s += vd * dot3 * C.m0 / rhob;
+            }
+            drho.add(s);
+          }
+          if ((long long)pta * (long long)ptb > 0) {
+            double Pfac = (Pha + eos_d(rhob, ptb, C)) / (rhoa * rhob);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
+          }
+          if (both) {
+            double cond = dotd<D>(dv, dx);
+            if (cond < 0) {
+              double dmu = C.h * cond / (r2 + C.eta2);
+              double dPi = (-C.alpha * C.c0 * dmu) / ((rhoa + rhob) * 0.5);
+#pragma unroll
+              for (int d = 0; d < D; ++d) ax[d] += -dPi * C.m0 * gw[d];
+            }
+            if (F.nu_on) {
+              double term = 4 * C.nu * C.m0 * dotd<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.eta2));
+#pragma unroll
+              for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+            }
+            if (F.s_on) {
+              double r = sqrt(r2);
+              if (r < C.st_2h) {
+                double scos = C.s * cos(C.st_c * r / C.st_2h) / (r + C.st_01h);
+#pragma unroll
+                for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.m0;
+              }
+            }
+          }
+          // fbmolforce! at xⁿ with (bound_D, bound_l) (src/sphproblem.jl:295)
+          if ((pta > 0) != (ptb > 0) && rn < C.fb_r0[1]) {
+            double q1 = C.fb_r0[1] / (rn + C.fb_eta2[1]);
+            double Fc = C.fb_d[1] * (powi_d(q1, 12) - powi_d(q1, 4)) / (r2n + C.fb_eta2[1]);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += dxn[d] * Fc;
+          }
+#pragma unroll
+          for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) av[d] = acc[d].get();
+      av[1] = av[1] - C.g;
+      drho_v = drho.get();
+      // symplectic_update! (src/gpukernels_sph.jl:1204-1245)
+      const double epsi = -(drho_v / rhoa) * dt;
+      double rv = B.RHO[p][a] * (2 - epsi) / (2 + epsi);
+      if (rv < C.rho0 && pta <= 1) rv = C.rho0;
+      B.RHO[q][a] = rv;
+      if (F.corr_on) B.RH2[a] = rv;
+      const double ml = pta > 1 ? 1.0 : 0.0;
+      double xn[3], vn[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double vv = B.V[p][d][a];
+        double nv = vv + av[d] * dt * ml;
+        double dd = (vv + nv) * 0.5 * dt;
+        double cd = dd;
+        if (F.need_cdx) {
+          cd = (rebuilt ? 0.0 : B.CDX[d][a]) + dd;
+          B.CDX[d][a] = cd;
+        }
+        red_cdx = fmax(red_cdx, fabs(cd)) + (cd != cd ? cd : 0.0);
+        double xv = xa[d] + dd;
+        xn[d] = xv;
+        vn[d] = nv;
+        B.V[q][d][a] = nv;
+        B.X[q][d][a] = xv;
+        if (F.corr_on) {
+          B.VH2[d][a] = nv;
+          B.XH2[d][a] = xv;
+        }
+      }
+      if (F.ts) {
+        red_acc = sqrt(dotd<D>(av, av));
+        if (F.visc_in_pass2) red_visc = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
+      }
+    } else {
+#pragma clang fp contract(fast)
+      const float dra = sha;
+      const float rhoa = C.f_rho0 + dra;
+      Pha = eos_f(dra, pta, C);
+      for (int k = 0; k < rows.n; ++k) {
+        for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
+          if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
+          const int32_t ptb = PT[b];
+          float dxn[3], dx[3], dv[3], gw[3];
+#pragma unroll
+          for (int d = 0; d < D; ++d) {
+            dxn[d] = float(xa[d] - X[d][b]);
+            dx[d] = float(xha[d] - XH[d][b]);
+            dv[d] = vha[d] - VH[d][b];
+          }
+          const float r2n = dotf<D>(dxn, dxn);
+          const float rn = sqrtf(r2n);
+          const float dwk_r = dW_f(rn * C.f_Hinv, C.f_dwnorm) / rn;
+#pragma unroll
+          for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
+          const float r2 = dotf<D>(dx, dx);
+          const float drb = RH[b];
+          const float rhob = C.f_rho0 + drb;
+          const bool both = pta > 0 && ptb > 0;
+          float ax[3] = {0.0f, 0.0f, 0.0f};
+          if (both) {
+            float dot3 = -dotf<D>(dx, gw);
+            float s = C.f_m0 * dotf<D>(dv, gw);
+            if (pta > 1) {
+              float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
+              float vd = C.f_DDTkh * ((drb - dra) - drhop) / (r2 + C.f_eta2);
+              s += vd * dot3 * C.f_m0 / rhob;
+            }
+            drho.add(s);
+          }
+          if ((long long)pta * (long long)ptb > 0) {
+            float Pfac = (Pha + eos_f(drb, ptb, C)) / (rhoa * rhob);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
+          }
+          if (both) {
+            float cond = dotf<D>(dv, dx);
+            if (cond < 0.0f) {
+              float dmu = C.f_h * cond / (r2 + C.f_eta2);
+              float dPi = (-C.f_alpha * C.f_c0 * dmu) / ((rhoa + rhob) * 0.5f);
+#pragma unroll
+              for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
+            }
+            if (F.nu_on) {
+              float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.f_eta2));
+#pragma unroll
+              for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+            }
+            if (F.s_on) {
+              float r = sqrtf(r2);
+              if (r < C.f_st_2h) {
+                float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
+#pragma unroll
+                for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.f_m0;
+              }
+            }
+          }
+          if ((pta > 0) != (ptb > 0) && rn < C.f_fb_r0[1]) {
+            float q1 = C.f_fb_r0[1] / (rn + C.f_fb_eta2[1]);
+            float q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
+            float Fc = C.f_fb_d[1] * (q4 * q8 - q4) / (r2n + C.f_fb_eta2[1]);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += dxn[d] * Fc;
+          }
+#pragma unroll
+          for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+        }
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) av[d] = acc[d].get();
+      av[1] = av[1] - C.f_g;
+      drho_v = drho.get();
+      // symplectic update on δρ: ρ(2-ε)/(2+ε) - ρ0 = (2δρ - ε(2ρ0 + δρ)) / (2 + ε)
+      const float fdt = float(dt);
+      const float epsi = -(drho_v / rhoa) * fdt;
+      const float dr_n = B.RHO[p][a];
+      float rv = (2.0f * dr_n - epsi * (2.0f * C.f_rho0 + dr_n)) / (2.0f + epsi);
+      if (rv < 0.0f && pta <= 1) rv = 0.0f;
+      B.RHO[q][a] = rv;
+      if (F.corr_on) B.RH2[a] = rv;
+      const float ml = pta > 1 ? 1.0f : 0.0f;
+      double xn[3], vn[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        float vv = B.V[p][d][a];
+        float nv = vv + av[d] * fdt * ml;
+        double dd = (double(vv) + double(nv)) * 0.5 * dt;
+        double cd = dd;
+        if (F.need_cdx) {
+          cd = (rebuilt ? 0.0 : B.CDX[d][a]) + dd;
+          B.CDX[d][a] = cd;
+        }
+        red_cdx = fmax(red_cdx, fabs(cd)) + (cd != cd ? cd : 0.0);
+        double xv = xa[d] + dd;
+        xn[d] = xv;
+        vn[d] = nv;
+        B.V[q][d][a] = nv;
+        B.X[q][d][a] = xv;
+        if (F.corr_on) {
+          B.VH2[d][a] = nv;
+          B.XH2[d][a] = xv;
+        }
+      }
+      if (F.ts) {
+        float a2 = dotf<D>(av, av);
+        red_acc = sqrt(double(a2));
+        if (F.visc_in_pass2) red_visc = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
+      }
+    }
+    if (F.want_out) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) B.ACC[d][a] = av[d];
+      B.DRHO[a] = drho_v;
+      B.PBUF[a] = Pha;
+    }
+  }
+  reduce_max(&ctl->red_cdx, red_cdx, active);
+  if (F.ts) {
+    reduce_max(&ctl->red_acc, red_acc, active);
+    if (F.visc_in_pass2) reduce_max(&ctl->red_visc, red_visc, active);
+  }
+}
+
+// ---- corrections after the update (src/sphproblem.jl:303-322) --------------
+// dpcreg! (src/gpukernels_sph.jl:1500-1553): ∑Δv_dpc into DV; P = EOS(ρ^{n+1}).
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_dpc_sum(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+#pragma clang fp contract(off)
+  const StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
+  const double dt = ctl->dt;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= B.n) return;
+  const double* X[3] = {B.X[q][0], B.X[q][1], B.X[q][D == 3 ? 2 : 1]};
+  const R* V[3] = {B.V[q][0], B.V[q][1], B.V[q][D == 3 ? 2 : 1]};
+  const R* RHO = B.RHO[q];
+  const int32_t* PT = B.PT[pp];
+  const double* XM[3];
+  membership_ptrs<D>(B, F, p, rebuilt, XM);
+  const int32_t pta = PT[a];
+  const double rhoa = std::is_same<R, double>::value ? double(RHO[a]) : C.rho0 + double(RHO[a]);
+  const double Pa = std::is_same<R, double>::value ? eos_d(rhoa, pta, C) : double(eos_f(float(RHO[a]), pta, C));
+  double sum[3] = {0.0, 0.0, 0.0};
+  const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  for (int k = 0; k < rows.n; ++k)
+    for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
+      if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
+      const int32_t ptb = PT[b];
+      if (!(pta > 0 && ptb > 0)) continue;
+      double dx[3], dv[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        dx[d] = X[d][a] - X[d][b];
+        dv[d] = double(V[d][a]) - double(V[d][b]);
+      }
+      const double rhob = std::is_same<R, double>::value ? double(RHO[b]) : C.rho0 + double(RHO[b]);
+      const double Pb = std::is_same<R, double>::value ? eos_d(rhob, ptb, C) : double(eos_f(float(RHO[b]), ptb, C));
+      const double r2 = dotd<D>(dx, dx);
+      const double r = sqrt(r2);
+      const double u = r * C.dpc_l0inv;
+      const double w = W_d(u, C.dpc_wnorm);
+      const double chi = sqrt(w * C.dpc_whinv);
+      const double kk = u < C.dpc_lim ? 1.0 : chi;
+      double lo = C.dpc_lambda * fabs(Pa + Pb), cap = C.dpc_lambda * C.dpc_pmax;
+      double pt = lo < cap ? lo : cap;
+      pt = pt > C.dpc_pmin ? pt : C.dpc_pmin;
+      const double Pb_dpc = chi * pt;
+      const double vr = dotd<D>(dv, dx);
+      if (vr < 0) {
+        double vrdr = vr / (r2 + C.dpc_eta2);
+#pragma unroll
+        for (int d = 0; d < D; ++d) sum[d] += kk * (-vrdr * dx[d]);
+      } else {
+        double tvar = 2 * dt * Pb_dpc / ((rhoa + rhob) * (r2 + C.dpc_eta2));
+#pragma unroll
+        for (int d = 0; d < D; ++d) sum[d] += tvar * dx[d];
+      }
+    }
+#pragma unroll
+  for (int d = 0; d < D; ++d) B.DV[d][a] = R(sum[d]);
+  if (F.want_out) B.PBUF[a] = R(Pa);
+}
+
+// update_dpcreg! (src/gpukernels_sph.jl:1617-1632)
+template <int D, typename R>
+__global__ void k_dpc_apply(SphBufs<R> B) {
+#pragma clang fp contract(off)
+  const StepCtl* ctl = B.ctl;
+  const int q = ctl->parity ^ 1, pp = ctl->pparity;
+  const double dt = ctl->dt;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= B.n || B.PT[pp][a] <= 1) return;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    R dv = B.DV[d][a];
+    B.V[q][d][a] = B.V[q][d][a] + dv;
+    B.X[q][d][a] = B.X[q][d][a] + double(dv) * dt;
+  }
+}
+
+// xsphcorr! + update_xsphcorr! (src/gpukernels_sph.jl:1757-1838): W at xⁿ.
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_xsph(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+#pragma clang fp contract(off)
+  const StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
+  const double dt = ctl->dt;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= B.n) return;
+  const int32_t* PT = B.PT[pp];
+  const int32_t pta = PT[a];
+  if (pta <= 1) return;
+  const double* XN[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
+  const R* V[3] = {B.V[q][0], B.V[q][1], B.V[q][D == 3 ? 2 : 1]};
+  const R* RHO = B.RHO[q];
+  const double* XM[3];
+  membership_ptrs<D>(B, F, p, rebuilt, XM);
+  const double rhoa = std::is_same<R, double>::value ? double(RHO[a]) : C.rho0 + double(RHO[a]);
+  double sum[3] = {0.0, 0.0, 0.0};
+  const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  for (int k = 0; k < rows.n; ++k)
+    for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
+      if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
+      if (PT[b] <= 1) continue;
+      double dx[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) dx[d] = XN[d][a] - XN[d][b];
+      const double r = sqrt(dotd<D>(dx, dx));
+      const double w = W_d(r * C.Hinv, C.wnorm);
+      const double rhob = std::is_same<R, double>::value ? double(RHO[b]) : C.rho0 + double(RHO[b]);
+      const double xs = C.xsph_c * w / (rhoa + rhob);
+#pragma unroll
+      for (int d = 0; d < D; ++d) sum[d] += (double(V[d][b]) - double(V[d][a])) * xs;
+    }
+#pragma unroll
+  for (int d = 0; d < D; ++d) B.X[q][d][a] = B.X[q][d][a] + sum[d] * dt;
+}
+
+// cspmcorr! (src/gpukernels_sph.jl:1693-1719): ρ = Σ m0 W / Σ m0 W / ρ_b, W at xⁿ.
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_cspm_sum(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+#pragma clang fp contract(off)
+  const StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= B.n) return;
+  const int32_t* PT = B.PT[pp];
+  const int32_t pta = PT[a];
+  const double* XN[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
+  const R* RHO = B.RHO[q];
+  const double* XM[3];
+  membership_ptrs<D>(B, F, p, rebuilt, XM);
+  double s1 = 0.0, s2 = 0.0;
+  if (pta > 0) {
+    const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
+    for (int k = 0; k < rows.n; ++k)
+      for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
+        if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
+        if (PT[b] <= 0) continue;
+        double dx[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) dx[d] = XN[d][a] - XN[d][b];
+        const double w = W_d(sqrt(dotd<D>(dx, dx)) * C.Hinv, C.wnorm);
+        const double rhob = std::is_same<R, double>::value ? double(RHO[b]) : C.rho0 + double(RHO[b]);
+        s1 += C.m0 * w;
+        s2 += w * C.m0 / rhob;
+      }
+  }
+  B.TMP[a] = s1 / s2;
+}
+
+template <typename R>
+__global__ void k_cspm_apply(SphBufs<R> B, double rho0) {
+  const StepCtl* ctl = B.ctl;
+  const int q = ctl->parity ^ 1;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= B.n) return;
+  double nr = B.TMP[a];
+  if (nr != nr) return;
+  if (std::is_same<R, double>::value) B.RHO[q][a] = R(nr);
+  else B.RHO[q][a] = R(nr - rho0);
+}
+
+// Δt viscous rate over the corrected state (src/gpukernels_sph.jl:1325-1334).
+template <int D, typename R>
+__global__ void k_dt_visc(SphBufs<R> B, SphConst C) {
+  StepCtl* ctl = B.ctl;
+  const int q = ctl->parity ^ 1;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool active = a < B.n;
+  double v = 0.0;
+  if (active) {
+    double vn[3], xn[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      vn[d] = double(B.V[q][d][a]);
+      xn[d] = B.X[q][d][a];
+    }
+    v = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
+  }
+  reduce_max(&ctl->red_visc, v, active);
+}
+
+__device__ __forceinline__ double jl_min(double a, double b) { return (a != a || b != b) ? a + b : (a < b ? a : b); }
+__device__ __forceinline__ double jl_max(double a, double b) { return (a != a || b != b) ? a + b : (a > b ? a : b); }
+
+// End of step: commits the new state, rebuild flag, time and Δt
+// (src/sphproblem.jl:325-335, src/gpukernels_sph.jl:1293-1323).
+__global__ void k_finalize(StepCtl* ctl, SphConst C, int ts, int corr_on) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double cdx = bits_to_double(ctl->red_cdx);
+  if (cdx > 0.9 * C.nui) ctl->rebuild = 1;
+  ctl->maxcdx_out = jl_max(ctl->maxcdx_out, cdx);
+  ctl->etime += ctl->dt;
+  if (ts) {
+    double amax = bits_to_double(ctl->red_acc);
+    double vmax = bits_to_double(ctl->red_visc);
+    double dt1 = sqrt(C.h / (3 * amax));
+    double dt2 = C.h / (C.c0 + vmax);
+    double dt = C.CFL * jl_min(dt1, dt2);
+    dt = jl_min(jl_max(dt, C.tl0), C.tl1);
+    ctl->dt = dt;
+  }
+  ctl->red_cdx = 0;
+  ctl->red_acc = 0;
+  ctl->red_visc = 0;
+  ctl->parity ^= 1;
+  ctl->synced = corr_on ? 0 : 1;
+  ctl->xrb_cur = 0;
+}
+
+// sph∑W! + sph∑∇W! over the current positions (src/gpukernels_sph.jl:139-156, :332-359).
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom G, int xm_mode) {
+#pragma clang fp contract(off)
+  const StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, pp = ctl->pparity;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= B.n) return;
+  const double* X[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
+  // membership positions: 0 = stored rebuild positions, 1 = the step's start
+  // positions (other buffer), 2 = the current positions
+  const int mp = xm_mode == 1 ? (p ^ 1) : p;
+  const double* XM[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) XM[k] = xm_mode == 0 ? B.XRB[k < D ? k : 1] : B.X[mp][k < D ? k : 1];
+  const int32_t* PT = B.PT[pp];
+  const bool pa = PT[a] > 0;
+  double sw = 0.0, sg[3] = {0.0, 0.0, 0.0};
+  const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  for (int k = 0; k < rows.n; ++k)
+    for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
+      if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
+      if (pa != (PT[b] > 0)) continue;
+      double dx[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) dx[d] = X[d][a] - X[d][b];
+      const double r = sqrt(dotd<D>(dx, dx));
+      const double u = r * C.Hinv;
+      sw += W_d(u, C.wnorm);
+      const double dwk_r = dW_d(u, C.dwnorm) / r;
+#pragma unroll
+      for (int d = 0; d < D; ++d) sg[d] += dx[d] * dwk_r;
+    }
+  B.SUMW[a] = sw;
+#pragma unroll
+  for (int d = 0; d < D; ++d) B.SUMGW[d][a] = sg[d];
+}
+
+// ------------------------------------------------------------ host launchers
+template <int D, typename R>
+void launch_reorder(const SphBufs<R>& B, const StepFlags& F, hipStream_t s) {
+  k_reorder<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, F);
+}
+template <int D, typename R>
+void launch_pass1(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
+  k_pass1<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
+}
+template <int D, typename R>
+void launch_pass2(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
+  k_pass2<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
+}
+template <int D, typename R>
+void launch_dpc(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
+  k_dpc_sum<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
+  k_dpc_apply<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B);
+}
+template <int D, typename R>
+void launch_xsph(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
+  k_xsph<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
+}
+template <int D, typename R>
+void launch_cspm(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
+  k_cspm_sum<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
+  k_cspm_apply<R><<<grid_for(B.n), kBlock, 0, s>>>(B, C.rho0);
+}
+template <int D, typename R>
+void launch_dt_visc(const SphBufs<R>& B, const SphConst& C, hipStream_t s) {
+  k_dt_visc<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C);
+}
+template <int D, typename R>
+void launch_diag(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int xm_mode, hipStream_t s) {
+  k_diag<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, xm_mode);
+}
+void launch_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb, hipStream_t s) {
+  k_post_rebuild<<<1, 64, 0, s>>>(ctl, maxcell, write_xrb);
+}
+void launch_finalize(StepCtl* ctl, const SphConst& C, int ts, int corr_on, hipStream_t s) {
+  k_finalize<<<1, 64, 0, s>>>(ctl, C, ts, corr_on);
+}
+
+#define GCLSPH_INSTANTIATE(D, R)                                                                               \
+  template void launch_reorder<D, R>(const SphBufs<R>&, const StepFlags&, hipStream_t);                        \
+  template void launch_pass1<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,      \
+                                   hipStream_t);                                                               \
+  template void launch_pass2<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,      \
+                                   hipStream_t);                                                               \
+  template void launch_dpc<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,        \
+                                 hipStream_t);                                                                 \
+  template void launch_xsph<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,       \
+                                  hipStream_t);                                                                \
+  template void launch_cspm<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,       \
+                                  hipStream_t);                                                                \
+  template void launch_dt_visc<D, R>(const SphBufs<R>&, const SphConst&, hipStream_t);                         \
+  template void launch_diag<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, int, hipStream_t);
+
+GCLSPH_INSTANTIATE(2, double)
+GCLSPH_INSTANTIATE(3, double)
+GCLSPH_INSTANTIATE(2, float)
+GCLSPH_INSTANTIATE(3, float)
+
+}  // namespace gclsph
