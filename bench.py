@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Throughput of the full SPH step (neighbour rebuild + both pair passes +
+symplectic update + CFL Δt) on one MI355X.
+
+Workload: the large 2D dam break — a 4.0 m x 4.0 m water column at dx = 0.002
+(2000 x 2000 = 4,000,000 fluid particles) in a 16 m x 8 m tank with 3 wall
+layers (47,991 particles), hydrostatic initial density, dist = H (a rebuild
+every step), adaptive CFL = 0.2 Δt.  A step is one `stepsolve!` iteration over
+the whole scene with all state resident in HBM.
+
+The headline `value` is the FP32-split path (FP32 pair sums with Kahan
+compensation, density as ρ-ρ0, FP64 positions and Δt); the FP64 path runs on
+the same GPU in the same invocation and is reported beside it.
+
+Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "gpucelllistsph.jl_amd"))
+
+import gclsph as G  # noqa: E402
+from gclsph import scenes as S  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(dim: int, split: bool) -> dict:
+    """Compulsory HBM bytes per particle for each stage (canonical field sizes:
+    positions 8·D, velocity 8·D or 4·D, density 8 or 4, ptype 4, cell key 4,
+    permutation 4); neighbour reuse through caches counts zero."""
+    x = 8 * dim
+    v = (4 if split else 8) * dim
+    r = 4 if split else 8
+    pt = 4
+    return {
+        "rebuild": 4 + x + 4 + 4 + 2 * (x + v + r + pt),  # bin: read x, write key; reorder: key+perm, state r/w
+        "pass1": (x + v + r + pt) + (x + v + r),  # read x, v, ρ, ptype; write xΔt½, vΔt½, ρΔt½
+        "pass2": (x + x + v + r + pt + v + r) + (x + v + r),  # read xⁿ, xΔt½, vΔt½, ρΔt½, ptype, vⁿ, ρⁿ; write x, v, ρ
+    }
+
+
+def run_gpu(scene, precision: str, steps: int, warmup: int):
+    cl = G.GPUCellList(scene.points, scene.cellsize, scene.dist)
+    prob = G.SPHProblem.from_params(cl, scene.params, scene.rho, scene.ptype, precision=precision)
+    tl = G.default_timelims(prob)
+    if warmup > 0:
+        G.stepsolve_(prob, warmup, timestepping=True, timelims=tl)
+    prob.set_timing(True)
+    t0 = time.perf_counter()
+    upd, _ = G.stepsolve_(prob, steps, timestepping=True, timelims=tl)
+    t1 = time.perf_counter()
+    timing = prob.timing()
+    prob.set_timing(False)
+    return {"seconds": t1 - t0, "updaten": upd, "timing": timing, "dt": prob.dt}
+
+
+def read_traffic(path: str):
+    """HBM bytes per dispatch of the dominant kernel, from the committed
+    rocprofv3 PMC summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
+    calibration), or None."""
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            if ":" in line and line.strip().startswith("traffic_bytes_per_dispatch"):
+                k, v = line.split(":", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except ValueError:
+                    pass
+    return vals.get("traffic_bytes_per_dispatch")
+
+
+def cpu_baseline(scene, seconds_budget: float = 60.0):
+    """The CPU oracle (the reference's algorithm restated in C++, one thread) on
+    one full step of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O  # test/check infrastructure, used here only as the CPU baseline
+
+    oc = O.CellList(scene.points, scene.cellsize, scene.dist)
+    sph = O.SPHProblem(oc, scene.params, scene.rho, scene.ptype)
+    t0 = time.perf_counter()
+    sph.stepsolve(1, True)
+    dt = time.perf_counter() - t0
+    return {"value": scene.n / dt, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle (C++ FP64 restatement of the reference step, 1 thread), 1 full step of the "
+                      f"same {scene.n}-particle scene including its neighbour rebuild: {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--dx", type=float, default=0.002)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pass2_f32split_traffic.txt"))
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the SPH step is built for a single MI355X "
+              "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    scene = S.config3(dx=args.dx)
+    n = scene.n
+    res32 = run_gpu(scene, "f32split", args.steps, args.warmup)
+    res64 = None if args.no_f64 else run_gpu(scene, "f64", args.steps, args.warmup)
+
+    value = n * args.steps / res32["seconds"]
+    ms_per_step = res32["seconds"] / args.steps * 1e3
+    bytes_pp = algorithmic_bytes(2, True)
+    stages = res32["timing"]
+    dom = max(("pass1", "pass2"), key=lambda k: stages[k][0])
+    dom_ms = stages[dom][0] / max(stages[dom][1], 1)
+    achieved = bytes_pp[dom] * n / (dom_ms * 1e-3) / 1e9
+    traffic = read_traffic(args.traffic_file) if dom == "pass2" else None
+    out = {
+        "metric": "particle-steps/s (full SPH step) FP32-split vs FP64 on 1 MI355X; HBM GB/s",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair sums + f64 positions/dt (FP32-split)",
+        "data": "synthetic",
+        "config": {
+            "workload": f"2D dam break dx={args.dx}: 4.0x4.0 m column ({int(scene.info['fluid'])} fluid) in a "
+                        f"16x8 m tank ({int(scene.info['boundary'])} wall), WendlandC2, dist=H, adaptive CFL=0.2",
+            "particles": n,
+            "rebuilds_in_timed_steps": res32["updaten"],
+            "precision": "FP32-split",
+        },
+        "stage_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in stages.items() if v[1] > 0},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_{dom}",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "bytes_per_particle": bytes_pp[dom],
+            "note": "pair passes are VALU-bound; algorithmic bytes / measured kernel time",
+        },
+    }
+    if res64 is not None:
+        v64 = n * args.steps / res64["seconds"]
+        out["f64"] = {"value": v64, "ms_per_step": res64["seconds"] / args.steps * 1e3,
+                      "stage_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in res64["timing"].items() if v[1] > 0}}
+        out["f32split_over_f64"] = value / v64
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -196,6 +196,7 @@ struct sph_s {
   StepCtl* ctl = nullptr;
   StepCtl hctl{};
   bool synced_prev = true;
+  bool p_from_dpc = false;  // get_pressure reads the DPC-stage pressure
   int xrb_mode = 0;  // membership positions for diagnostics / pair list: 0 XRB, 1 other, 2 current
   bool timing = false;
   double stage_ms[5] = {0, 0, 0, 0, 0};
@@ -265,7 +266,11 @@ struct sph_s {
     B.RH2 = dalloc<R>(n);
     B.DRHO = dalloc<R>(n);
     B.PBUF = dalloc<R>(n);
+    B.PN = dalloc<R>(n);
+    B.PH = dalloc<R>(n);
     B.TMP = dalloc<double>(n);
+    B.nblk = int(grid_for(n));
+    B.RED = dalloc<double>(3 * int64_t(B.nblk));
     B.SUMW = dalloc<double>(n);
     B.start = sb.start;
     B.skey = sb.skey;
@@ -420,12 +425,12 @@ struct sph_s {
   SphBufs<R>& bufs();
 
   template <int D, typename R>
-  void rebuild_stage(const StepFlags& F) {
+  void rebuild_stage(const SphConst& C, const StepFlags& F) {
     SphBufs<R>& B = bufs<R>();
     const void* x0[3] = {B.X[0][0], B.X[0][1], B.X[0][2]};
     const void* x1[3] = {B.X[1][0], B.X[1][1], B.X[1][2]};
     bin_sort(G, 64, x0, n, false, sb, &ctl->rebuild, stream, x1, &ctl->parity);
-    launch_reorder<D, R>(B, F, stream);
+    launch_reorder<D, R>(B, C, F, stream);
     GCL_HIP(hipGetLastError());
     launch_post_rebuild(ctl, sb.maxcell, F.write_xrb, stream);
     GCL_HIP(hipGetLastError());
@@ -461,7 +466,7 @@ struct sph_s {
       F.corr_on = corr_on ? 1 : 0;
       F.want_out = it == nsteps ? 1 : 0;
       hipEvent_t e0 = mark_begin();
-      rebuild_stage<D, R>(F);
+      rebuild_stage<D, R>(C, F);
       mark_end(0, e0);
       e0 = mark_begin();
       launch_pass1<D, R>(B, C, G, F, stream);
@@ -481,12 +486,15 @@ struct sph_s {
         mark_end(3, e0);
       }
       e0 = mark_begin();
-      launch_finalize(ctl, C, F.ts, F.corr_on, stream);
+      launch_finalize(ctl, B.RED, B.nblk, C, F.ts, F.corr_on, stream);
       GCL_HIP(hipGetLastError());
       mark_end(4, e0);
       synced_prev = !corr_on;
     }
-    if (nsteps > 0) xrb_mode = need_xrb ? 0 : 1;
+    if (nsteps > 0) {
+      xrb_mode = need_xrb ? 0 : 1;
+      p_from_dpc = dpc_on;
+    }
     launch_diag<D, R>(B, C, G, xrb_mode, stream);
     GCL_HIP(hipGetLastError());
     read_ctl();
@@ -537,7 +545,7 @@ struct sph_s {
     StepFlags F{};
     F.synced_prev = synced_prev ? 1 : 0;
     F.write_xrb = 1;
-    rebuild_stage<D, R>(F);
+    rebuild_stage<D, R>(make_const(0.0, 0.0), F);
     // per-particle arrays outside the reorder follow the new order
     auto permute_d = [&](double* a) {
       double* t = B.TMP;
@@ -557,6 +565,7 @@ struct sph_s {
     permute_d(B.SUMW);
     permute_r(B.DRHO);
     permute_r(B.PBUF);
+    permute_r(B.PH);
     if (!synced_prev) {  // the reorder moved the half-step state into XH/VH/RH
       for (int k = 0; k < D; ++k) {
         GCL_HIP(hipMemcpyAsync(B.XH2[k], B.XH[k], 8 * size_t(n), hipMemcpyDeviceToDevice, stream));
@@ -1015,6 +1024,9 @@ int sph_create(gcl_t system, int prec, const double* params, const double* rho, 
     P->hctl.dt = P->par[SPH_DT];
     P->hctl.etime = P->par[SPH_ETIME];
     P->write_ctl();
+    if (prec == GCLSPH_F32SPLIT) launch_init_pn<float>(P->Bf, P->make_const(0.0, 0.0), P->stream);
+    else launch_init_pn<double>(P->Bd, P->make_const(0.0, 0.0), P->stream);
+    GCL_HIP(hipGetLastError());
     GCL_HIP(hipStreamSynchronize(P->stream));
     P->xrb_mode = 0;
     system->attached = P.get();
@@ -1092,9 +1104,9 @@ int sph_get_field(sph_t h, int field, double* out) {
         if (sp) put(unsort<float>(h->Bf.RHO[p], id, n, h->stream, rho0), 0);
         else put(unsort<double>(h->Bd.RHO[p], id, n, h->stream), 0);
         break;
-      case SPH_FIELD_P:
-        if (sp) put(unsort<float>(h->Bf.PBUF, id, n, h->stream), 0);
-        else put(unsort<double>(h->Bd.PBUF, id, n, h->stream), 0);
+      case SPH_FIELD_P:  // the last pressure! call: pass 2's half-step EOS, or EOS(ρ) before DPC
+        if (sp) put(unsort<float>(h->p_from_dpc ? h->Bf.PBUF : h->Bf.PH, id, n, h->stream), 0);
+        else put(unsort<double>(h->p_from_dpc ? h->Bd.PBUF : h->Bd.PH, id, n, h->stream), 0);
         break;
       case SPH_FIELD_ACC:
         if (sp) vec_r(h->Bf.ACC, 0.0);

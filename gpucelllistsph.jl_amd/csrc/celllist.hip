@@ -66,7 +66,7 @@ struct XPtrs {
 template <typename T, int D>
 __global__ void k_cellmap(XPtrs xp, XPtrs xalt, const int* sel, int64_t n, CellGeom g, int use_off64,
                           int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ cnt,
-                          const int* flag) {
+                          const int* flag, int aggregate) {
   if (flag && *flag == 0) return;
   int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -75,8 +75,26 @@ __global__ void k_cellmap(XPtrs xp, XPtrs xalt, const int* sel, int64_t n, CellG
 #pragma unroll
   for (int k = 0; k < D; ++k) c[k] = cell_coord<T>(static_cast<const T*>(src.p[k])[i], g, k, use_off64);
   int64_t lin = int64_t(c[0] - 1) + g.grid[0] * (int64_t(c[1] - 1) + g.grid[1] * int64_t(c[2] - 1));
-  key[i] = int32_t(lin);
-  slot[i] = atomicAdd(&cnt[lin], 1);
+  const int32_t k = int32_t(lin);
+  key[i] = k;
+  if (!aggregate) {
+    slot[i] = atomicAdd(&cnt[lin], 1);
+    return;
+  }
+  // Input already in (old) cell order: lanes of a wave mostly share a few
+  // cells, so one atomic per distinct cell of the wave serves all its lanes.
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(1);
+  while (todo) {
+    const int leader = __ffsll((long long)todo) - 1;
+    const int32_t lk = __shfl(k, leader, 64);
+    const unsigned long long peers = __ballot(k == lk) & todo;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(&cnt[lk], __popcll(peers));
+    base = __shfl(base, leader, 64);
+    if (k == lk) slot[i] = base + __popcll(peers & ((1ull << lane) - 1));
+    todo &= ~peers;
+  }
 }
 
 __global__ void k_scatter(const int32_t* __restrict__ key, const int32_t* __restrict__ slot,
@@ -120,9 +138,10 @@ __global__ void k_pair_count(XPtrs xsp, const int32_t* __restrict__ start, const
   if (s >= n) return;
   const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
                    static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
-  Rows r = stencil_rows<D>(g, start, skey[s]);
+  const Rows<D> r = stencil_rows<D>(g, start, skey[s]);
   int32_t c = 0;
-  for (int k = 0; k < r.n; ++k) {
+#pragma unroll
+  for (int k = 0; k < Rows<D>::N; ++k) {
     int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
     for (int32_t t = b; t < r.e[k]; ++t) c += dist2_exact<T, D>(x, s, t) < dist2max;
   }
@@ -139,9 +158,10 @@ __global__ void k_pair_emit(XPtrs xsp, const int32_t* __restrict__ start, const 
   if (o >= capacity) return;
   const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
                    static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
-  Rows r = stencil_rows<D>(g, start, skey[s]);
+  const Rows<D> r = stencil_rows<D>(g, start, skey[s]);
   int32_t is = id[s] + 1;
-  for (int k = 0; k < r.n; ++k) {
+#pragma unroll
+  for (int k = 0; k < Rows<D>::N; ++k) {
     int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
     for (int32_t t = b; t < r.e[k]; ++t) {
       if (dist2_exact<T, D>(x, s, t) < dist2max) {
@@ -165,9 +185,9 @@ void bin_sort_t(const CellGeom& g, const void* const* x, int64_t n, bool use_off
   int64_t nc = g.ncells();
   fill_i32(sb.cnt, nc + 1, 0, flag, s);
   if (g.dim == 2)
-    k_cellmap<T, 2><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag);
+    k_cellmap<T, 2><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, sel != nullptr);
   else
-    k_cellmap<T, 3><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag);
+    k_cellmap<T, 3><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, sel != nullptr);
   GCL_HIP(hipGetLastError());
   fill_i32(sb.maxcell, 1, 0, flag, s);
   exclusive_scan_i32(sb.cnt, sb.start, nc, sb.scan_tmp, flag, s, sb.maxcell);

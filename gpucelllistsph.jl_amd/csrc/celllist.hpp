@@ -47,35 +47,44 @@ struct PairBuffers {
   void release();
 };
 
-// Stencil rows of the sorted item s: for each (dy, dz) the contiguous sorted
-// range of cells [ix-1, ix+1] (clamped to the grid).
+// Stencil rows of a sorted item in cell c: for each (dy, dz) in {-1,0,1}^(D-1)
+// the contiguous sorted range of cells [ix-1, ix+1] clamped to the grid (cells
+// are ordered x fastest).  Rows outside the grid are empty (b == e).  The arrays
+// have a fixed size and are filled by unrolled code, so they stay in registers.
+template <int D>
 struct Rows {
-  int n;
-  int32_t b[9], e[9];
+  static constexpr int N = D == 2 ? 3 : 9;
+  int32_t b[N], e[N];
 };
 
 template <int D>
-__device__ __forceinline__ Rows stencil_rows(const CellGeom& g, const int32_t* __restrict__ start, int32_t c) {
-  Rows r;
-  r.n = 0;
-  int64_t gx = g.grid[0], gy = g.grid[1];
-  int64_t ix = c % gx;
-  int64_t rest = c / gx;
-  int64_t iy = rest % gy;
-  int64_t iz = rest / gy;
-  int64_t xl = ix > 0 ? ix - 1 : 0, xh = ix + 1 < gx ? ix + 1 : gx - 1;
-  int64_t zlo = D == 3 ? (iz > 0 ? iz - 1 : 0) : 0;
-  int64_t zhi = D == 3 ? (iz + 1 < g.grid[2] ? iz + 1 : g.grid[2] - 1) : 0;
-  for (int64_t z = zlo; z <= zhi; ++z)
-    for (int64_t y = (iy > 0 ? iy - 1 : 0); y <= (iy + 1 < gy ? iy + 1 : gy - 1); ++y) {
-      int64_t row = gx * (y + gy * z);
-      r.b[r.n] = start[row + xl];
-      r.e[r.n] = start[row + xh + 1];
-      ++r.n;
-    }
+__device__ __forceinline__ Rows<D> stencil_rows(const CellGeom& g, const int32_t* __restrict__ start, int32_t c) {
+  Rows<D> r;
+  const int64_t gx = g.grid[0], gy = g.grid[1];
+  const int64_t ix = c % gx;
+  const int64_t rest = c / gx;
+  const int64_t iy = rest % gy;
+  const int64_t iz = rest / gy;
+  const int64_t xl = ix > 0 ? ix - 1 : 0, xh = ix + 1 < gx ? ix + 1 : gx - 1;
+#pragma unroll
+  for (int k = 0; k < Rows<D>::N; ++k) {
+    const int64_t y = iy + (k % 3) - 1;
+    const int64_t z = D == 3 ? iz + (k / 3) - 1 : 0;
+    const bool in = y >= 0 && y < gy && (D == 2 || (z >= 0 && z < g.grid[2]));
+    const int64_t row = gx * (y + gy * z);
+    r.b[k] = in ? start[row + xl] : 0;
+    r.e[k] = in ? start[row + xh + 1] : 0;
+  }
   return r;
 }
 
+template <int N, typename T>
+__device__ __forceinline__ T select_k(const T (&v)[N], int k) {
+  T r = v[0];
+#pragma unroll
+  for (int i = 1; i < N; ++i) r = k == i ? v[i] : r;
+  return r;
+}
 
 // Bins positions `x` (D pointers, input order, element type double or float)
 // into cells, and writes the sorted order (perm, skey, start).  The order

@@ -60,18 +60,36 @@ __global__ void k_tile_sums(const int32_t* __restrict__ in, int64_t n, int64_t* 
   int64_t tot;
   block_exclusive_scan<int64_t>(s, &tot);
   if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+  if (max_out) {  // tile maximum, reduced by k_scan_tile_sums
+    __shared__ int32_t wmx[kScanThreads / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+      int32_t t = __shfl_xor(mx, o, 64);
+      mx = t > mx ? t : mx;
+    }
+    if ((threadIdx.x & 63) == 0) wmx[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int32_t m = wmx[0];
+      for (int w = 1; w < kScanThreads / 64; ++w) m = wmx[w] > m ? wmx[w] : m;
+      reinterpret_cast<int32_t*>(tsum + gridDim.x + 2)[blockIdx.x] = m;
+    }
+  }
+}
+
+// Single block: exclusive scan of tile sums in place; tsum[nt] = grand total;
+// with max_out, also the maximum of the tile maxima stored after the sums.
+__global__ void k_scan_tile_sums(int64_t* __restrict__ tsum, int64_t nt, const int* flag, int32_t* max_out) {
+  if (flag && *flag == 0) return;
   if (max_out) {
+    const int32_t* tm = reinterpret_cast<const int32_t*>(tsum + nt + 2);
+    int32_t mx = 0;
+    for (int64_t i = threadIdx.x; i < nt; i += kScanThreads) mx = tm[i] > mx ? tm[i] : mx;
     for (int o = 32; o > 0; o >>= 1) {
       int32_t t = __shfl_xor(mx, o, 64);
       mx = t > mx ? t : mx;
     }
     if ((threadIdx.x & 63) == 0) atomicMax(max_out, mx);
   }
-}
-
-// Single block: exclusive scan of tile sums in place; tsum[nt] = grand total.
-__global__ void k_scan_tile_sums(int64_t* __restrict__ tsum, int64_t nt, const int* flag) {
-  if (flag && *flag == 0) return;
   int64_t per = (nt + kScanThreads - 1) / kScanThreads;
   int64_t b = int64_t(threadIdx.x) * per, e = b + per < nt ? b + per : nt;
   int64_t s = 0;
@@ -124,7 +142,7 @@ void scan_impl(const int32_t* in, O* out, int64_t n, int64_t* tmp, const int* fl
   if (nt < 1) nt = 1;
   k_tile_sums<<<unsigned(nt), kScanThreads, 0, s>>>(in, n, tmp, flag, max_out);
   GCL_HIP(hipGetLastError());
-  k_scan_tile_sums<<<1, kScanThreads, 0, s>>>(tmp, nt, flag);
+  k_scan_tile_sums<<<1, kScanThreads, 0, s>>>(tmp, nt, flag, max_out);
   GCL_HIP(hipGetLastError());
   k_tile_scan<O><<<unsigned(nt), kScanThreads, 0, s>>>(in, out, n, tmp, nt, flag);
   GCL_HIP(hipGetLastError());
@@ -132,7 +150,11 @@ void scan_impl(const int32_t* in, O* out, int64_t n, int64_t* tmp, const int* fl
 
 }  // namespace
 
-size_t scan_tmp_elems(int64_t n) { return size_t((n + kTile - 1) / kTile + 2); }
+// tile sums (nt + 2 int64) followed by the tile maxima (nt int32)
+size_t scan_tmp_elems(int64_t n) {
+  int64_t nt = (n + kTile - 1) / kTile;
+  return size_t(nt + 2 + (nt + 1) / 2 + 1);
+}
 
 void exclusive_scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* tmp, const int* flag,
                         hipStream_t s, int32_t* max_out) {

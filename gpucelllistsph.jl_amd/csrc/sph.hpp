@@ -64,6 +64,8 @@ struct SphBufs {
   R* ACC[3];
   R* DRHO;
   R* PBUF;
+  R* PN;  // EOS of the current density (written with every density update)
+  R* PH;  // EOS of the half-step density (written by pass 1)
   R* DV[3];
   double* TMP;
   double* SUMW;
@@ -73,10 +75,14 @@ struct SphBufs {
   const int32_t* perm;
   const int32_t* maxcell;
   StepCtl* ctl;
+  double* RED;  // per-block partial maxima: [|cdx|, |a|, visc] x nblk
+  int nblk;
 };
 
 template <int D, typename R>
-void launch_reorder(const SphBufs<R>& B, const StepFlags& F, hipStream_t s);
+void launch_reorder(const SphBufs<R>& B, const SphConst& C, const StepFlags& F, hipStream_t s);
+template <typename R>
+void launch_init_pn(const SphBufs<R>& B, const SphConst& C, hipStream_t s);
 template <int D, typename R>
 void launch_pass1(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s);
 template <int D, typename R>
@@ -92,6 +98,6 @@ void launch_dt_visc(const SphBufs<R>& B, const SphConst& C, hipStream_t s);
 template <int D, typename R>
 void launch_diag(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int xm_mode, hipStream_t s);
 void launch_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb, hipStream_t s);
-void launch_finalize(StepCtl* ctl, const SphConst& C, int ts, int corr_on, hipStream_t s);
+void launch_finalize(StepCtl* ctl, const double* red, int nblk, const SphConst& C, int ts, int corr_on, hipStream_t s);
 
 }  // namespace gclsph

@@ -180,9 +180,75 @@ __device__ __forceinline__ void membership_ptrs(const SphBufs<R>& B, const StepF
   for (int k = 0; k < 3; ++k) XM[k] = cur ? B.X[p][k < D ? k : 1] : B.XRB[k < D ? k : 1];
 }
 
-__device__ __forceinline__ void reduce_max(unsigned long long* slot, double v, bool active) {
+// Visits the neighbour-list partners b of particle a: candidates of the 3^D
+// stencil rows passing the membership test.  Candidates are taken in windows of
+// 64 per row; the members of a window are first recorded as bit masks and then
+// visited, so all lanes of a wave run the (long) pair body together instead of
+// idling on the roughly two thirds of candidates that are not partners.
+template <int D, typename F>
+__device__ __forceinline__ void for_each_neighbor(const Rows<D>& rows, int64_t a, const double* const* XM,
+                                                  double dist2, F&& body) {
+  constexpr int NR = Rows<D>::N;
+  const double xa0 = XM[0][a], xa1 = XM[1][a], xa2 = D == 3 ? XM[2][a] : 0.0;
+  for (int32_t w0 = 0;; w0 += 64) {
+    unsigned long long m[NR];
+    int32_t base[NR];
+    bool more = false;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int32_t lo = rows.b[k] + w0;
+      const int32_t hi = rows.e[k] < lo + 64 ? rows.e[k] : lo + 64;
+      unsigned long long mk = 0;
+      for (int32_t b = lo; b < hi; ++b) {
+#pragma clang fp contract(off)
+        const double d0 = xa0 - XM[0][b];
+        const double d1 = xa1 - XM[1][b];
+        double r2 = d0 * d0 + d1 * d1;
+        if (D == 3) {
+          const double d2 = xa2 - XM[2][b];
+          r2 = r2 + d2 * d2;
+        }
+        if (r2 < dist2 && b != a) mk |= 1ull << (b - lo);
+      }
+      m[k] = mk;
+      base[k] = lo;
+      more |= rows.e[k] > lo + 64;
+    }
+    int k = 0;
+    unsigned long long c = m[0];
+    int32_t cb = base[0];
+    for (;;) {
+      while (c == 0 && k < NR - 1) {
+        ++k;
+        c = select_k(m, k);
+        cb = select_k(base, k);
+      }
+      if (c == 0) break;
+      const int bit = __builtin_ctzll(c);
+      c &= c - 1;
+      body(cb + bit);
+    }
+    if (!more) break;
+  }
+}
+
+__device__ __forceinline__ double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+// Block maximum (NaN wins) written to out[blockIdx.x]; k_finalize reduces the
+// per-block values, so no same-address atomics are needed.  Every thread of
+// the block must call it.
+__device__ __forceinline__ void block_max(double v, bool active, double* out) {
+  __shared__ double wm[kBlock / 64];
   double m = wave_max(active ? v : 0.0);
-  if ((threadIdx.x & 63) == 0) atomic_max_nonneg(slot, m);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = wm[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) r = nanmax(r, wm[w]);
+    out[blockIdx.x] = r;
+  }
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------ kernels
@@ -205,7 +271,7 @@ __global__ void k_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_x
 
 // New sorted position s takes the state of old position perm[s].
 template <int D, typename R>
-__global__ void k_reorder(SphBufs<R> B, StepFlags F) {
+__global__ void k_reorder(SphBufs<R> B, SphConst C, StepFlags F) {
   const StepCtl* ctl = B.ctl;
   if (ctl->rebuild == 0) return;
   int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -223,9 +289,13 @@ __global__ void k_reorder(SphBufs<R> B, StepFlags F) {
       B.VH[k][s] = B.VH2[k][src];
     }
   }
-  B.RHO[q][s] = B.RHO[p][src];
+  const R rv = B.RHO[p][src];
+  const int32_t pt = B.PT[pp][src];
+  B.RHO[q][s] = rv;
+  if constexpr (std::is_same<R, double>::value) B.PN[s] = eos_d(rv, pt, C);
+  else B.PN[s] = eos_f(rv, pt, C);
   if (!F.synced_prev) B.RH[s] = B.RH2[src];
-  B.PT[pq][s] = B.PT[pp][src];
+  B.PT[pq][s] = pt;
   B.ID[pq][s] = B.ID[pp][src];
 }
 
@@ -256,73 +326,70 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
   const int32_t pta = PT[a];
   Sum<R> acc[3], drho;
 
-  const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
   if constexpr (std::is_same<R, double>::value) {
     const double rhoa = sa;
-    const double Pa = eos_d(rhoa, pta, C);
-    for (int k = 0; k < rows.n; ++k) {
-      for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
-        if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
-        const int32_t ptb = PT[b];
-        double dx[3], dv[3], gw[3];
+    const double Pa = B.PN[a];
+    for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+      const int32_t ptb = PT[b];
+      double dx[3], dv[3], gw[3];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          dx[d] = xa[d] - X[d][b];
-          dv[d] = va[d] - V[d][b];
-        }
-        const double r2 = dotd<D>(dx, dx);
-        const double r = sqrt(r2);
-        const double u = r * C.Hinv;
-        const double dwk_r = dW_d(u, C.dwnorm) / r;
-#pragma unroll
-        for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
-        const double rhob = RHO[b];
-        const bool both = pta > 0 && ptb > 0;
-        double ax[3] = {0.0, 0.0, 0.0};
-        if (both) {  // ∂ρ∂tDDT! (:579-625)
-          double dot3 = -dotd<D>(dx, gw);
-          double s = C.m0 * dotd<D>(dv, gw);
-          if (pta > 1) {
-            double drhop = C.rho0 * pf7(1 + C.DDTgz * dx[1], C.gamma, C.ginv) - C.rho0;
-            double vd = C.DDTkh * (rhob - rhoa - drhop) / (r2 + C.eta2);
-            s += vd * dot3 * C.m0 / rhob;
-          }
-          drho.add(s);
-        }
-        if ((long long)pta * (long long)ptb > 0) {  // ∂v∂t! (:755-772)
-          double Pfac = (Pa + eos_d(rhob, ptb, C)) / (rhoa * rhob);
-#pragma unroll
-          for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
-        }
-        if (both) {  // ∂v∂t_av! (:869-897)
-          double cond = dotd<D>(dv, dx);
-          if (cond < 0) {
-            double dmu = C.h * cond / (r2 + C.eta2);
-            double dPi = (-C.alpha * C.c0 * dmu) / ((rhoa + rhob) * 0.5);
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += -dPi * C.m0 * gw[d];
-          }
-          if (F.nu_on) {  // ∂v∂t_visc! (:967-980)
-            double term = 4 * C.nu * C.m0 * dotd<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.eta2));
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
-          }
-          if (F.s_on && r < C.st_2h) {  // ∂v∂tpF! (:1387-1397)
-            double scos = C.s * cos(C.st_c * r / C.st_2h) / (r + C.st_01h);
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.m0;
-          }
-        }
-        if ((pta > 0) != (ptb > 0) && r < C.fb_r0[0]) {  // fbmolforce! (:1891-1901)
-          double q1 = C.fb_r0[0] / (r + C.fb_eta2[0]);
-          double Fc = C.fb_d[0] * (powi_d(q1, 12) - powi_d(q1, 4)) / (r2 + C.fb_eta2[0]);
-#pragma unroll
-          for (int d = 0; d < D; ++d) ax[d] += dx[d] * Fc;
-        }
-#pragma unroll
-        for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+      for (int d = 0; d < D; ++d) {
+        dx[d] = xa[d] - X[d][b];
+        dv[d] = va[d] - V[d][b];
       }
-    }
+      const double r2 = dotd<D>(dx, dx);
+      const double r = sqrt(r2);
+      const double u = r * C.Hinv;
+      const double dwk_r = dW_d(u, C.dwnorm) / r;
+#pragma unroll
+      for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
+      const double rhob = RHO[b];
+      const bool both = pta > 0 && ptb > 0;
+      double ax[3] = {0.0, 0.0, 0.0};
+      if (both) {  // ∂ρ∂tDDT! (:579-625)
+        double dot3 = -dotd<D>(dx, gw);
+        double s = C.m0 * dotd<D>(dv, gw);
+        if (pta > 1) {
+          double drhop = C.rho0 * pf7(1 + C.DDTgz * dx[1], C.gamma, C.ginv) - C.rho0;
+          double vd = C.DDTkh * (rhob - rhoa - drhop) / (r2 + C.eta2);
+          s += vd * dot3 * C.m0 / rhob;
+        }
+        drho.add(s);
+      }
+      if ((long long)pta * (long long)ptb > 0) {  // ∂v∂t! (:755-772)
+        double Pfac = (Pa + B.PN[b]) / (rhoa * rhob);
+#pragma unroll
+        for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
+      }
+      if (both) {  // ∂v∂t_av! (:869-897)
+        double cond = dotd<D>(dv, dx);
+        if (cond < 0) {
+          double dmu = C.h * cond / (r2 + C.eta2);
+          double dPi = (-C.alpha * C.c0 * dmu) / ((rhoa + rhob) * 0.5);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -dPi * C.m0 * gw[d];
+        }
+        if (F.nu_on) {  // ∂v∂t_visc! (:967-980)
+          double term = 4 * C.nu * C.m0 * dotd<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.eta2));
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+        }
+        if (F.s_on && r < C.st_2h) {  // ∂v∂tpF! (:1387-1397)
+          double scos = C.s * cos(C.st_c * r / C.st_2h) / (r + C.st_01h);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.m0;
+        }
+      }
+      if ((pta > 0) != (ptb > 0) && r < C.fb_r0[0]) {  // fbmolforce! (:1891-1901)
+        double q1 = C.fb_r0[0] / (r + C.fb_eta2[0]);
+        double Fc = C.fb_d[0] * (powi_d(q1, 12) - powi_d(q1, 4)) / (r2 + C.fb_eta2[0]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) ax[d] += dx[d] * Fc;
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+    });
     // ∂v∂t_addgrav!, then the half-step updates (:270-274)
     double av[3];
 #pragma unroll
@@ -349,6 +416,7 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
     double rv = rh0 + drho.get() * hdt;
     if (rv < C.rho0 && pta < 0) rv = C.rho0;
     B.RH[a] = rv;
+    B.PH[a] = eos_d(rv, pta, C);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       double vv = vh0[d];
@@ -361,72 +429,70 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
 #pragma clang fp contract(fast)
     const float dra = sa;
     const float rhoa = C.f_rho0 + dra;
-    const float Pa = eos_f(dra, pta, C);
-    for (int k = 0; k < rows.n; ++k) {
-      for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
-        if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
-        const int32_t ptb = PT[b];
-        float dx[3], dv[3], gw[3];
+    const float Pa = B.PN[a];
+    for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+      const int32_t ptb = PT[b];
+      float dx[3], dv[3], gw[3];
 #pragma unroll
-        for (int d = 0; d < D; ++d) {
-          dx[d] = float(xa[d] - X[d][b]);
-          dv[d] = va[d] - V[d][b];
-        }
-        const float r2 = dotf<D>(dx, dx);
-        const float r = sqrtf(r2);
-        const float u = r * C.f_Hinv;
-        const float dwk_r = dW_f(u, C.f_dwnorm) / r;
-#pragma unroll
-        for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
-        const float drb = RHO[b];
-        const float rhob = C.f_rho0 + drb;
-        const bool both = pta > 0 && ptb > 0;
-        float ax[3] = {0.0f, 0.0f, 0.0f};
-        if (both) {
-          float dot3 = -dotf<D>(dx, gw);
-          float s = C.f_m0 * dotf<D>(dv, gw);
-          if (pta > 1) {
-            float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
-            float vd = C.f_DDTkh * ((drb - dra) - drhop) / (r2 + C.f_eta2);
-            s += vd * dot3 * C.f_m0 / rhob;
-          }
-          drho.add(s);
-        }
-        if ((long long)pta * (long long)ptb > 0) {
-          float Pfac = (Pa + eos_f(drb, ptb, C)) / (rhoa * rhob);
-#pragma unroll
-          for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
-        }
-        if (both) {
-          float cond = dotf<D>(dv, dx);
-          if (cond < 0.0f) {
-            float dmu = C.f_h * cond / (r2 + C.f_eta2);
-            float dPi = (-C.f_alpha * C.f_c0 * dmu) / ((rhoa + rhob) * 0.5f);
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
-          }
-          if (F.nu_on) {
-            float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.f_eta2));
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
-          }
-          if (F.s_on && r < C.f_st_2h) {
-            float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.f_m0;
-          }
-        }
-        if ((pta > 0) != (ptb > 0) && r < C.f_fb_r0[0]) {
-          float q1 = C.f_fb_r0[0] / (r + C.f_fb_eta2[0]);
-          float q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
-          float Fc = C.f_fb_d[0] * (q4 * q8 - q4) / (r2 + C.f_fb_eta2[0]);
-#pragma unroll
-          for (int d = 0; d < D; ++d) ax[d] += dx[d] * Fc;
-        }
-#pragma unroll
-        for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+      for (int d = 0; d < D; ++d) {
+        dx[d] = float(xa[d] - X[d][b]);
+        dv[d] = va[d] - V[d][b];
       }
-    }
+      const float r2 = dotf<D>(dx, dx);
+      const float r = sqrtf(r2);
+      const float u = r * C.f_Hinv;
+      const float dwk_r = dW_f(u, C.f_dwnorm) / r;
+#pragma unroll
+      for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
+      const float drb = RHO[b];
+      const float rhob = C.f_rho0 + drb;
+      const bool both = pta > 0 && ptb > 0;
+      float ax[3] = {0.0f, 0.0f, 0.0f};
+      if (both) {
+        float dot3 = -dotf<D>(dx, gw);
+        float s = C.f_m0 * dotf<D>(dv, gw);
+        if (pta > 1) {
+          float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
+          float vd = C.f_DDTkh * ((drb - dra) - drhop) / (r2 + C.f_eta2);
+          s += vd * dot3 * C.f_m0 / rhob;
+        }
+        drho.add(s);
+      }
+      if ((long long)pta * (long long)ptb > 0) {
+        float Pfac = (Pa + B.PN[b]) / (rhoa * rhob);
+#pragma unroll
+        for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
+      }
+      if (both) {
+        float cond = dotf<D>(dv, dx);
+        if (cond < 0.0f) {
+          float dmu = C.f_h * cond / (r2 + C.f_eta2);
+          float dPi = (-C.f_alpha * C.f_c0 * dmu) / ((rhoa + rhob) * 0.5f);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
+        }
+        if (F.nu_on) {
+          float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.f_eta2));
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+        }
+        if (F.s_on && r < C.f_st_2h) {
+          float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.f_m0;
+        }
+      }
+      if ((pta > 0) != (ptb > 0) && r < C.f_fb_r0[0]) {
+        // wall force in FP64: (r0/r)^12 overflows FP32 for near-coincident pairs
+        double q1 = C.fb_r0[0] / (double(r) + C.fb_eta2[0]);
+        double q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
+        double Fc = C.fb_d[0] * (q4 * q8 - q4) / (double(r2) + C.fb_eta2[0]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) ax[d] += float(double(dx[d]) * Fc);
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+    });
     float av[3];
 #pragma unroll
     for (int d = 0; d < D; ++d) av[d] = acc[d].get();
@@ -453,6 +519,7 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
     float rv = rh0 + drho.get() * hdt;
     if (rv < 0.0f && pta < 0) rv = 0.0f;
     B.RH[a] = rv;
+    B.PH[a] = eos_f(rv, pta, C);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       float vv = vh0[d];
@@ -492,81 +559,78 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
     const R sha = RH[a];
     const int32_t pta = PT[a];
     Sum<R> acc[3], drho;
-    const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
+    const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
     R av[3];
     R Pha, drho_v;
     if constexpr (std::is_same<R, double>::value) {
       const double rhoa = sha;
-      Pha = eos_d(rhoa, pta, C);
-      for (int k = 0; k < rows.n; ++k) {
-        for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
-          if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
-          const int32_t ptb = PT[b];
-          double dxn[3], dx[3], dv[3], gw[3];
+      Pha = B.PH[a];
+      for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+        const int32_t ptb = PT[b];
+        double dxn[3], dx[3], dv[3], gw[3];
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            dxn[d] = xa[d] - X[d][b];
-            dx[d] = xha[d] - XH[d][b];
-            dv[d] = vha[d] - VH[d][b];
-          }
-          // ∇W from pass 1 (positions xⁿ, src/sphproblem.jl:246,281)
-          const double r2n = dotd<D>(dxn, dxn);
-          const double rn = sqrt(r2n);
-          const double dwk_r = dW_d(rn * C.Hinv, C.dwnorm) / rn;
-#pragma unroll
-          for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
-          const double r2 = dotd<D>(dx, dx);
-          const double rhob = RH[b];
-          const bool both = pta > 0 && ptb > 0;
-          double ax[3] = {0.0, 0.0, 0.0};
-          if (both) {
-            double dot3 = -dotd<D>(dx, gw);
-            double s = C.m0 * dotd<D>(dv, gw);
-            if (pta > 1) {
-              double drhop = C.rho0 * pf7(1 + C.DDTgz * dx[1], C.gamma, C.ginv) - C.rho0;
-              double vd = C.DDTkh * (rhob - rhoa - drhop) / (r2 + C.eta2);
-              s += vd * dot3 * C.m0 / rhob;
-            }
-            drho.add(s);
-          }
-          if ((long long)pta * (long long)ptb > 0) {
-            double Pfac = (Pha + eos_d(rhob, ptb, C)) / (rhoa * rhob);
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
-          }
-          if (both) {
-            double cond = dotd<D>(dv, dx);
-            if (cond < 0) {
-              double dmu = C.h * cond / (r2 + C.eta2);
-              double dPi = (-C.alpha * C.c0 * dmu) / ((rhoa + rhob) * 0.5);
-#pragma unroll
-              for (int d = 0; d < D; ++d) ax[d] += -dPi * C.m0 * gw[d];
-            }
-            if (F.nu_on) {
-              double term = 4 * C.nu * C.m0 * dotd<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.eta2));
-#pragma unroll
-              for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
-            }
-            if (F.s_on) {
-              double r = sqrt(r2);
-              if (r < C.st_2h) {
-                double scos = C.s * cos(C.st_c * r / C.st_2h) / (r + C.st_01h);
-#pragma unroll
-                for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.m0;
-              }
-            }
-          }
-          // fbmolforce! at xⁿ with (bound_D, bound_l) (src/sphproblem.jl:295)
-          if ((pta > 0) != (ptb > 0) && rn < C.fb_r0[1]) {
-            double q1 = C.fb_r0[1] / (rn + C.fb_eta2[1]);
-            double Fc = C.fb_d[1] * (powi_d(q1, 12) - powi_d(q1, 4)) / (r2n + C.fb_eta2[1]);
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += dxn[d] * Fc;
-          }
-#pragma unroll
-          for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+        for (int d = 0; d < D; ++d) {
+          dxn[d] = xa[d] - X[d][b];
+          dx[d] = xha[d] - XH[d][b];
+          dv[d] = vha[d] - VH[d][b];
         }
-      }
+        // ∇W from pass 1 (positions xⁿ, src/sphproblem.jl:246,281)
+        const double r2n = dotd<D>(dxn, dxn);
+        const double rn = sqrt(r2n);
+        const double dwk_r = dW_d(rn * C.Hinv, C.dwnorm) / rn;
+#pragma unroll
+        for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
+        const double r2 = dotd<D>(dx, dx);
+        const double rhob = RH[b];
+        const bool both = pta > 0 && ptb > 0;
+        double ax[3] = {0.0, 0.0, 0.0};
+        if (both) {
+          double dot3 = -dotd<D>(dx, gw);
+          double s = C.m0 * dotd<D>(dv, gw);
+          if (pta > 1) {
+            double drhop = C.rho0 * pf7(1 + C.DDTgz * dx[1], C.gamma, C.ginv) - C.rho0;
+            double vd = C.DDTkh * (rhob - rhoa - drhop) / (r2 + C.eta2);
+            s += vd * dot3 * C.m0 / rhob;
+          }
+          drho.add(s);
+        }
+        if ((long long)pta * (long long)ptb > 0) {
+          double Pfac = (Pha + B.PH[b]) / (rhoa * rhob);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
+        }
+        if (both) {
+          double cond = dotd<D>(dv, dx);
+          if (cond < 0) {
+            double dmu = C.h * cond / (r2 + C.eta2);
+            double dPi = (-C.alpha * C.c0 * dmu) / ((rhoa + rhob) * 0.5);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += -dPi * C.m0 * gw[d];
+          }
+          if (F.nu_on) {
+            double term = 4 * C.nu * C.m0 * dotd<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.eta2));
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+          }
+          if (F.s_on) {
+            double r = sqrt(r2);
+            if (r < C.st_2h) {
+              double scos = C.s * cos(C.st_c * r / C.st_2h) / (r + C.st_01h);
+#pragma unroll
+              for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.m0;
+            }
+          }
+        }
+        // fbmolforce! at xⁿ with (bound_D, bound_l) (src/sphproblem.jl:295)
+        if ((pta > 0) != (ptb > 0) && rn < C.fb_r0[1]) {
+          double q1 = C.fb_r0[1] / (rn + C.fb_eta2[1]);
+          double Fc = C.fb_d[1] * (powi_d(q1, 12) - powi_d(q1, 4)) / (r2n + C.fb_eta2[1]);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += dxn[d] * Fc;
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+      });
 #pragma unroll
       for (int d = 0; d < D; ++d) av[d] = acc[d].get();
       av[1] = av[1] - C.g;
@@ -576,6 +640,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
       double rv = B.RHO[p][a] * (2 - epsi) / (2 + epsi);
       if (rv < C.rho0 && pta <= 1) rv = C.rho0;
       B.RHO[q][a] = rv;
+      B.PN[a] = eos_d(rv, pta, C);
       if (F.corr_on) B.RH2[a] = rv;
       const double ml = pta > 1 ? 1.0 : 0.0;
       double xn[3], vn[3];
@@ -608,76 +673,74 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
 #pragma clang fp contract(fast)
       const float dra = sha;
       const float rhoa = C.f_rho0 + dra;
-      Pha = eos_f(dra, pta, C);
-      for (int k = 0; k < rows.n; ++k) {
-        for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
-          if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
-          const int32_t ptb = PT[b];
-          float dxn[3], dx[3], dv[3], gw[3];
+      Pha = B.PH[a];
+      for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+        const int32_t ptb = PT[b];
+        float dxn[3], dx[3], dv[3], gw[3];
 #pragma unroll
-          for (int d = 0; d < D; ++d) {
-            dxn[d] = float(xa[d] - X[d][b]);
-            dx[d] = float(xha[d] - XH[d][b]);
-            dv[d] = vha[d] - VH[d][b];
-          }
-          const float r2n = dotf<D>(dxn, dxn);
-          const float rn = sqrtf(r2n);
-          const float dwk_r = dW_f(rn * C.f_Hinv, C.f_dwnorm) / rn;
-#pragma unroll
-          for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
-          const float r2 = dotf<D>(dx, dx);
-          const float drb = RH[b];
-          const float rhob = C.f_rho0 + drb;
-          const bool both = pta > 0 && ptb > 0;
-          float ax[3] = {0.0f, 0.0f, 0.0f};
-          if (both) {
-            float dot3 = -dotf<D>(dx, gw);
-            float s = C.f_m0 * dotf<D>(dv, gw);
-            if (pta > 1) {
-              float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
-              float vd = C.f_DDTkh * ((drb - dra) - drhop) / (r2 + C.f_eta2);
-              s += vd * dot3 * C.f_m0 / rhob;
-            }
-            drho.add(s);
-          }
-          if ((long long)pta * (long long)ptb > 0) {
-            float Pfac = (Pha + eos_f(drb, ptb, C)) / (rhoa * rhob);
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
-          }
-          if (both) {
-            float cond = dotf<D>(dv, dx);
-            if (cond < 0.0f) {
-              float dmu = C.f_h * cond / (r2 + C.f_eta2);
-              float dPi = (-C.f_alpha * C.f_c0 * dmu) / ((rhoa + rhob) * 0.5f);
-#pragma unroll
-              for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
-            }
-            if (F.nu_on) {
-              float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.f_eta2));
-#pragma unroll
-              for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
-            }
-            if (F.s_on) {
-              float r = sqrtf(r2);
-              if (r < C.f_st_2h) {
-                float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
-#pragma unroll
-                for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.f_m0;
-              }
-            }
-          }
-          if ((pta > 0) != (ptb > 0) && rn < C.f_fb_r0[1]) {
-            float q1 = C.f_fb_r0[1] / (rn + C.f_fb_eta2[1]);
-            float q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
-            float Fc = C.f_fb_d[1] * (q4 * q8 - q4) / (r2n + C.f_fb_eta2[1]);
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += dxn[d] * Fc;
-          }
-#pragma unroll
-          for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+        for (int d = 0; d < D; ++d) {
+          dxn[d] = float(xa[d] - X[d][b]);
+          dx[d] = float(xha[d] - XH[d][b]);
+          dv[d] = vha[d] - VH[d][b];
         }
-      }
+        const float r2n = dotf<D>(dxn, dxn);
+        const float rn = sqrtf(r2n);
+        const float dwk_r = dW_f(rn * C.f_Hinv, C.f_dwnorm) / rn;
+#pragma unroll
+        for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
+        const float r2 = dotf<D>(dx, dx);
+        const float drb = RH[b];
+        const float rhob = C.f_rho0 + drb;
+        const bool both = pta > 0 && ptb > 0;
+        float ax[3] = {0.0f, 0.0f, 0.0f};
+        if (both) {
+          float dot3 = -dotf<D>(dx, gw);
+          float s = C.f_m0 * dotf<D>(dv, gw);
+          if (pta > 1) {
+            float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
+            float vd = C.f_DDTkh * ((drb - dra) - drhop) / (r2 + C.f_eta2);
+            s += vd * dot3 * C.f_m0 / rhob;
+          }
+          drho.add(s);
+        }
+        if ((long long)pta * (long long)ptb > 0) {
+          float Pfac = (Pha + B.PH[b]) / (rhoa * rhob);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
+        }
+        if (both) {
+          float cond = dotf<D>(dv, dx);
+          if (cond < 0.0f) {
+            float dmu = C.f_h * cond / (r2 + C.f_eta2);
+            float dPi = (-C.f_alpha * C.f_c0 * dmu) / ((rhoa + rhob) * 0.5f);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
+          }
+          if (F.nu_on) {
+            float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.f_eta2));
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+          }
+          if (F.s_on) {
+            float r = sqrtf(r2);
+            if (r < C.f_st_2h) {
+              float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
+#pragma unroll
+              for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.f_m0;
+            }
+          }
+        }
+        if ((pta > 0) != (ptb > 0) && rn < C.f_fb_r0[1]) {
+          // wall force in FP64: (r0/r)^12 overflows FP32 for near-coincident pairs
+          double q1 = C.fb_r0[1] / (double(rn) + C.fb_eta2[1]);
+          double q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
+          double Fc = C.fb_d[1] * (q4 * q8 - q4) / (double(r2n) + C.fb_eta2[1]);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += float(double(dxn[d]) * Fc);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+      });
 #pragma unroll
       for (int d = 0; d < D; ++d) av[d] = acc[d].get();
       av[1] = av[1] - C.f_g;
@@ -689,6 +752,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
       float rv = (2.0f * dr_n - epsi * (2.0f * C.f_rho0 + dr_n)) / (2.0f + epsi);
       if (rv < 0.0f && pta <= 1) rv = 0.0f;
       B.RHO[q][a] = rv;
+      B.PN[a] = eos_f(rv, pta, C);
       if (F.corr_on) B.RH2[a] = rv;
       const float ml = pta > 1 ? 1.0f : 0.0f;
       double xn[3], vn[3];
@@ -726,10 +790,10 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
       B.PBUF[a] = Pha;
     }
   }
-  reduce_max(&ctl->red_cdx, red_cdx, active);
+  block_max(red_cdx, active, B.RED);
   if (F.ts) {
-    reduce_max(&ctl->red_acc, red_acc, active);
-    if (F.visc_in_pass2) reduce_max(&ctl->red_visc, red_visc, active);
+    block_max(red_acc, active, B.RED + B.nblk);
+    if (F.visc_in_pass2) block_max(red_visc, active, B.RED + 2 * B.nblk);
   }
 }
 
@@ -751,14 +815,12 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SphBufs<R> B, SphConst C, CellG
   membership_ptrs<D>(B, F, p, rebuilt, XM);
   const int32_t pta = PT[a];
   const double rhoa = std::is_same<R, double>::value ? double(RHO[a]) : C.rho0 + double(RHO[a]);
-  const double Pa = std::is_same<R, double>::value ? eos_d(rhoa, pta, C) : double(eos_f(float(RHO[a]), pta, C));
+  const double Pa = double(B.PN[a]);
   double sum[3] = {0.0, 0.0, 0.0};
-  const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
-  for (int k = 0; k < rows.n; ++k)
-    for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
-      if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
+  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
       const int32_t ptb = PT[b];
-      if (!(pta > 0 && ptb > 0)) continue;
+      if (!(pta > 0 && ptb > 0)) return;
       double dx[3], dv[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -766,7 +828,7 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SphBufs<R> B, SphConst C, CellG
         dv[d] = double(V[d][a]) - double(V[d][b]);
       }
       const double rhob = std::is_same<R, double>::value ? double(RHO[b]) : C.rho0 + double(RHO[b]);
-      const double Pb = std::is_same<R, double>::value ? eos_d(rhob, ptb, C) : double(eos_f(float(RHO[b]), ptb, C));
+      const double Pb = double(B.PN[b]);
       const double r2 = dotd<D>(dx, dx);
       const double r = sqrt(r2);
       const double u = r * C.dpc_l0inv;
@@ -787,7 +849,7 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SphBufs<R> B, SphConst C, CellG
 #pragma unroll
         for (int d = 0; d < D; ++d) sum[d] += tvar * dx[d];
       }
-    }
+  });
 #pragma unroll
   for (int d = 0; d < D; ++d) B.DV[d][a] = R(sum[d]);
   if (F.want_out) B.PBUF[a] = R(Pa);
@@ -829,11 +891,9 @@ __global__ __launch_bounds__(256) void k_xsph(SphBufs<R> B, SphConst C, CellGeom
   membership_ptrs<D>(B, F, p, rebuilt, XM);
   const double rhoa = std::is_same<R, double>::value ? double(RHO[a]) : C.rho0 + double(RHO[a]);
   double sum[3] = {0.0, 0.0, 0.0};
-  const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
-  for (int k = 0; k < rows.n; ++k)
-    for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
-      if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
-      if (PT[b] <= 1) continue;
+  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+      if (PT[b] <= 1) return;
       double dx[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) dx[d] = XN[d][a] - XN[d][b];
@@ -843,7 +903,7 @@ __global__ __launch_bounds__(256) void k_xsph(SphBufs<R> B, SphConst C, CellGeom
       const double xs = C.xsph_c * w / (rhoa + rhob);
 #pragma unroll
       for (int d = 0; d < D; ++d) sum[d] += (double(V[d][b]) - double(V[d][a])) * xs;
-    }
+  });
 #pragma unroll
   for (int d = 0; d < D; ++d) B.X[q][d][a] = B.X[q][d][a] + sum[d] * dt;
 }
@@ -864,11 +924,9 @@ __global__ __launch_bounds__(256) void k_cspm_sum(SphBufs<R> B, SphConst C, Cell
   membership_ptrs<D>(B, F, p, rebuilt, XM);
   double s1 = 0.0, s2 = 0.0;
   if (pta > 0) {
-    const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
-    for (int k = 0; k < rows.n; ++k)
-      for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
-        if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
-        if (PT[b] <= 0) continue;
+    const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
+    for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+        if (PT[b] <= 0) return;
         double dx[3];
 #pragma unroll
         for (int d = 0; d < D; ++d) dx[d] = XN[d][a] - XN[d][b];
@@ -876,21 +934,41 @@ __global__ __launch_bounds__(256) void k_cspm_sum(SphBufs<R> B, SphConst C, Cell
         const double rhob = std::is_same<R, double>::value ? double(RHO[b]) : C.rho0 + double(RHO[b]);
         s1 += C.m0 * w;
         s2 += w * C.m0 / rhob;
-      }
+    });
   }
   B.TMP[a] = s1 / s2;
 }
 
 template <typename R>
-__global__ void k_cspm_apply(SphBufs<R> B, double rho0) {
+__global__ void k_cspm_apply(SphBufs<R> B, SphConst C) {
   const StepCtl* ctl = B.ctl;
-  const int q = ctl->parity ^ 1;
+  const int q = ctl->parity ^ 1, pp = ctl->pparity;
   const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (a >= B.n) return;
   double nr = B.TMP[a];
   if (nr != nr) return;
-  if (std::is_same<R, double>::value) B.RHO[q][a] = R(nr);
-  else B.RHO[q][a] = R(nr - rho0);
+  const int32_t pt = B.PT[pp][a];
+  if constexpr (std::is_same<R, double>::value) {
+    B.RHO[q][a] = nr;
+    B.PN[a] = eos_d(nr, pt, C);
+  } else {
+    float d = float(nr - C.rho0);
+    B.RHO[q][a] = d;
+    B.PN[a] = eos_f(d, pt, C);
+  }
+}
+
+// PN = EOS(current density) for every particle (after construction).
+template <typename R>
+__global__ void k_init_pn(SphBufs<R> B, SphConst C) {
+  const StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, pp = ctl->pparity;
+  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= B.n) return;
+  const R rv = B.RHO[p][a];
+  const int32_t pt = B.PT[pp][a];
+  if constexpr (std::is_same<R, double>::value) B.PN[a] = eos_d(rv, pt, C);
+  else B.PN[a] = eos_f(rv, pt, C);
 }
 
 // Δt viscous rate over the corrected state (src/gpukernels_sph.jl:1325-1334).
@@ -910,7 +988,7 @@ __global__ void k_dt_visc(SphBufs<R> B, SphConst C) {
     }
     v = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
   }
-  reduce_max(&ctl->red_visc, v, active);
+  block_max(v, active, B.RED + 2 * B.nblk);
 }
 
 __device__ __forceinline__ double jl_min(double a, double b) { return (a != a || b != b) ? a + b : (a < b ? a : b); }
@@ -918,15 +996,30 @@ __device__ __forceinline__ double jl_max(double a, double b) { return (a != a ||
 
 // End of step: commits the new state, rebuild flag, time and Δt
 // (src/sphproblem.jl:325-335, src/gpukernels_sph.jl:1293-1323).
-__global__ void k_finalize(StepCtl* ctl, SphConst C, int ts, int corr_on) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  double cdx = bits_to_double(ctl->red_cdx);
+__global__ void k_finalize(StepCtl* ctl, const double* __restrict__ red, int nblk, SphConst C, int ts, int corr_on) {
+  double loc[3] = {0.0, 0.0, 0.0};
+  const int ns = ts ? 3 : 1;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+    for (int k = 0; k < ns; ++k) loc[k] = nanmax(loc[k], red[k * nblk + b]);
+  double tot[3];
+  for (int k = 0; k < 3; ++k) {
+    __shared__ double wm[kBlock / 64];
+    double m = wave_max(loc[k]);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    double r = wm[0];
+    for (int w = 1; w < kBlock / 64; ++w) r = nanmax(r, wm[w]);
+    tot[k] = r;
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  double cdx = tot[0];
   if (cdx > 0.9 * C.nui) ctl->rebuild = 1;
   ctl->maxcdx_out = jl_max(ctl->maxcdx_out, cdx);
   ctl->etime += ctl->dt;
   if (ts) {
-    double amax = bits_to_double(ctl->red_acc);
-    double vmax = bits_to_double(ctl->red_visc);
+    double amax = tot[1];
+    double vmax = tot[2];
     double dt1 = sqrt(C.h / (3 * amax));
     double dt2 = C.h / (C.c0 + vmax);
     double dt = C.CFL * jl_min(dt1, dt2);
@@ -959,11 +1052,9 @@ __global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom
   const int32_t* PT = B.PT[pp];
   const bool pa = PT[a] > 0;
   double sw = 0.0, sg[3] = {0.0, 0.0, 0.0};
-  const Rows rows = stencil_rows<D>(G, B.start, B.skey[a]);
-  for (int k = 0; k < rows.n; ++k)
-    for (int32_t b = rows.b[k]; b < rows.e[k]; ++b) {
-      if (b == a || !member<D>(XM, a, b, C.dist2)) continue;
-      if (pa != (PT[b] > 0)) continue;
+  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+      if (pa != (PT[b] > 0)) return;
       double dx[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) dx[d] = X[d][a] - X[d][b];
@@ -973,7 +1064,7 @@ __global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom
       const double dwk_r = dW_d(u, C.dwnorm) / r;
 #pragma unroll
       for (int d = 0; d < D; ++d) sg[d] += dx[d] * dwk_r;
-    }
+  });
   B.SUMW[a] = sw;
 #pragma unroll
   for (int d = 0; d < D; ++d) B.SUMGW[d][a] = sg[d];
@@ -981,9 +1072,15 @@ __global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom
 
 // ------------------------------------------------------------ host launchers
 template <int D, typename R>
-void launch_reorder(const SphBufs<R>& B, const StepFlags& F, hipStream_t s) {
-  k_reorder<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, F);
+void launch_reorder(const SphBufs<R>& B, const SphConst& C, const StepFlags& F, hipStream_t s) {
+  k_reorder<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, F);
 }
+template <typename R>
+void launch_init_pn(const SphBufs<R>& B, const SphConst& C, hipStream_t s) {
+  k_init_pn<R><<<grid_for(B.n), kBlock, 0, s>>>(B, C);
+}
+template void launch_init_pn<double>(const SphBufs<double>&, const SphConst&, hipStream_t);
+template void launch_init_pn<float>(const SphBufs<float>&, const SphConst&, hipStream_t);
 template <int D, typename R>
 void launch_pass1(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
   k_pass1<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
@@ -1003,8 +1100,7 @@ void launch_xsph(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, cons
 }
 template <int D, typename R>
 void launch_cspm(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
-  k_cspm_sum<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
-  k_cspm_apply<R><<<grid_for(B.n), kBlock, 0, s>>>(B, C.rho0);
+  k_cspm_sum<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);  k_cspm_apply<R><<<grid_for(B.n), kBlock, 0, s>>>(B, C);
 }
 template <int D, typename R>
 void launch_dt_visc(const SphBufs<R>& B, const SphConst& C, hipStream_t s) {
@@ -1017,12 +1113,13 @@ void launch_diag(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int 
 void launch_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb, hipStream_t s) {
   k_post_rebuild<<<1, 64, 0, s>>>(ctl, maxcell, write_xrb);
 }
-void launch_finalize(StepCtl* ctl, const SphConst& C, int ts, int corr_on, hipStream_t s) {
-  k_finalize<<<1, 64, 0, s>>>(ctl, C, ts, corr_on);
+void launch_finalize(StepCtl* ctl, const double* red, int nblk, const SphConst& C, int ts, int corr_on,
+                     hipStream_t s) {
+  k_finalize<<<1, kBlock, 0, s>>>(ctl, red, nblk, C, ts, corr_on);
 }
 
 #define GCLSPH_INSTANTIATE(D, R)                                                                               \
-  template void launch_reorder<D, R>(const SphBufs<R>&, const StepFlags&, hipStream_t);                        \
+  template void launch_reorder<D, R>(const SphBufs<R>&, const SphConst&, const StepFlags&, hipStream_t);                        \
   template void launch_pass1<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,      \
                                    hipStream_t);                                                               \
   template void launch_pass2<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,      \

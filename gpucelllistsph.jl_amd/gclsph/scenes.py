@@ -152,9 +152,26 @@ def tank_boundary_2d(dx: float, width: float, height: float, layers: int = 3) ->
     return np.concatenate(xs), np.concatenate(ys)
 
 
+def fixture_wall_types(x: np.ndarray, y: np.ndarray, dx: float, width: float, layers: int = 3) -> np.ndarray:
+    """Wall typing of the reference's fixtures (test/input/BoundaryPoints_Dp0.02.csv):
+    the layer next to the fluid is ptype 1 (takes part in density, pressure and
+    viscosity), the outer layers are ptype 0 (only the repulsive wall force)."""
+    inner = (np.isclose(y, (layers - 1) * dx) & (x >= (layers - 1) * dx - 1e-9) & (x <= width - (layers - 1) * dx + 1e-9))
+    inner |= (np.isclose(x, (layers - 1) * dx) | np.isclose(x, width - (layers - 1) * dx)) & (y >= (layers - 1) * dx - 1e-9)
+    return np.where(inner, 1, 0).astype(np.int32)
+
+
+def _assert_distinct(points: Sequence[np.ndarray], dx: float):
+    key = np.stack([np.round(p / dx * 4).astype(np.int64) for p in points], axis=1)
+    if len(np.unique(key, axis=0)) != len(key):
+        raise ValueError("scene has coincident particles")
+
+
 def dam_break_2d(dx: float, fluid_w: float, fluid_h: float, tank_w: float, tank_h: float, layers: int = 3,
-                 boundary_ptype: int = -1, fluid_ptype: int = 2, dt: float = 1e-5, dist_factor: float = 1.0,
+                 boundary_ptype="fixture", fluid_ptype: int = 2, dt: float = 1e-5, dist_factor: float = 1.0,
                  name: str = "") -> Scene:
+    """boundary_ptype: an int for every wall particle, or "fixture" for the
+    reference fixtures' typing (inner layer 1, outer layers 0)."""
     p, dist = standard_params(dx, 2, dt, dist_factor)
     nfx = int(round(fluid_w / dx))
     nfy = int(round(fluid_h / dx))
@@ -166,7 +183,12 @@ def dam_break_2d(dx: float, fluid_w: float, fluid_h: float, tank_w: float, tank_
     x = np.concatenate([FX, bx])
     y = np.concatenate([FY, by])
     nf = FX.shape[0]
-    ptype = np.concatenate([np.full(nf, fluid_ptype, np.int32), np.full(bx.shape[0], boundary_ptype, np.int32)])
+    if isinstance(boundary_ptype, str):
+        wt = fixture_wall_types(bx, by, dx, tank_w, layers)
+    else:
+        wt = np.full(bx.shape[0], boundary_ptype, np.int32)
+    ptype = np.concatenate([np.full(nf, fluid_ptype, np.int32), wt])
+    _assert_distinct((x, y), dx)
     surface = float(FY.max())
     rho = np.concatenate([hydrostatic_rho(FY, surface, p.rho0, p.g, p.c0, p.gamma), np.full(bx.shape[0], p.rho0)])
     return Scene(2, (x, y), rho, ptype, p, dist, (p.H * dist_factor, p.H * dist_factor), None, name,
@@ -175,9 +197,14 @@ def dam_break_2d(dx: float, fluid_w: float, fluid_h: float, tank_w: float, tank_
 
 def dam_break_3d(dx: float, fluid: Sequence[float], tank: Sequence[float], layers: int = 3,
                  boundary_ptype: int = -1, fluid_ptype: int = 2, dt: float = 1e-5, name: str = "") -> Scene:
-    """fluid = (length x, height y, depth z); tank = (length, height, depth); y vertical."""
+    """fluid = (length x, height y, depth z); tank = (length, height, depth); y vertical.
+    The walls lie inside the tank bounds (as in 2D), so a tank dimension smaller
+    than the fluid block plus both wall stacks is widened to fit it."""
     p, dist = standard_params(dx, 3, dt)
     nfx, nfy, nfz = (int(round(a / dx)) for a in fluid)
+    tank = list(tank)
+    tank[0] = max(tank[0], (nfx - 1 + 2 * layers) * dx)
+    tank[2] = max(tank[2], (nfz - 1 + 2 * layers) * dx)
     fx = _lattice(nfx, layers * dx, dx)
     fy = _lattice(nfy, layers * dx, dx)
     fz = _lattice(nfz, layers * dx, dx)
@@ -193,6 +220,7 @@ def dam_break_3d(dx: float, fluid: Sequence[float], tank: Sequence[float], layer
     bx, by, bz = gx[shell], gy[shell], gz[shell]
     nf = FX.shape[0]
     pts = (np.concatenate([FX, bx]), np.concatenate([FY, by]), np.concatenate([FZ, bz]))
+    _assert_distinct(pts, dx)
     ptype = np.concatenate([np.full(nf, fluid_ptype, np.int32), np.full(bx.shape[0], boundary_ptype, np.int32)])
     surface = float(FY.max())
     rho = np.concatenate([hydrostatic_rho(FY, surface, p.rho0, p.g, p.c0, p.gamma), np.full(bx.shape[0], p.rho0)])
@@ -214,7 +242,7 @@ def jitter(points: Tuple[np.ndarray, ...], seed: int, step: int, sigma: float = 
 # ---------------------------------------------------------------------------
 def config0() -> Scene:
     """2D dam break: 1.0 x 2.0 m fluid column at dx=0.02 (50x100) in a 4 x 3 m tank, ptype -1 walls."""
-    return dam_break_2d(0.02, 1.0, 2.0, 4.0, 3.0, name="dam-break-2d dx=0.02 (5,000 fluid)")
+    return dam_break_2d(0.02, 1.0, 2.0, 4.0, 3.0, boundary_ptype=-1, name="dam-break-2d dx=0.02 (5,000 fluid)")
 
 
 def config1_points(seed: int = 12345, n: int = 200_000) -> Tuple[np.ndarray, np.ndarray]:
@@ -226,7 +254,8 @@ CONFIG1_DIST = 0.016
 
 
 def config2(dx: float = 0.01) -> Scene:
-    """3D dam break: 1.0 x 1.0 x 0.5 m fluid block (x, y-up, z) in a 3.2 x 1.5 x 0.5 m tank."""
+    """3D dam break: 1.0 x 1.0 x 0.5 m fluid block (x, y-up, z) in a 3.2 x 1.5 x 0.5 m tank
+    (widened in z to hold the 0.5 m block between its wall stacks), ptype -1 walls."""
     sc = dam_break_3d(dx, (1.0, 1.0, 0.5), (3.2, 1.5, 0.5), name=f"dam-break-3d dx={dx}")
     sc.params.nu = 0.2
     sc.params.s = 0.05
@@ -238,7 +267,8 @@ def config2(dx: float = 0.01) -> Scene:
 
 
 def config3(dx: float = 0.002, fluid: float = 4.0) -> Scene:
-    """Large 2D dam break: 4 x 4 m column at dx=0.002 (4,000,000 fluid) in a 16 x 8 m tank."""
+    """Large 2D dam break: 4 x 4 m column at dx=0.002 (4,000,000 fluid) in a 16 x 8 m tank,
+    walls typed as in the reference's fixtures (inner layer 1, outer layers 0)."""
     return dam_break_2d(dx, fluid, fluid, 16.0, 8.0, name=f"dam-break-2d dx={dx} ({fluid}x{fluid} m column)")
 
 

@@ -1,0 +1,165 @@
+"""GPU cell list vs the CPU oracle: pair lists are bit-exact as sorted (i, j)
+sets, together with every reference-visible field (grid, offset, cellpnum,
+pcell, pairsn, capacity, mppcell, мaxpairs)."""
+import math
+import os
+import warnings
+
+import numpy as np
+import pytest
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def as_set(pairs):
+    return np.unique(np.asarray(pairs), axis=0)
+
+
+def assert_same_celllist(g, o, check_capacity=True):
+    inf = o.info()
+    assert g.grid == inf["grid"]
+    assert g.offset == inf["offset"]
+    assert g.cs == inf["cs"]
+    assert g.pairsn == inf["pairsn"]
+    if check_capacity:
+        assert g.capacity == inf["capacity"]
+        assert g.mppcell == inf["mppcell"]
+    pc, cp = o.cells()
+    assert np.array_equal(g.cellpnum, cp)
+    assert np.array_equal(g.pcell, pc)
+    assert np.array_equal(as_set(__import__("gclsph").neighborlistview(g)), as_set(o.neighborlistview()))
+
+
+def test_config0(G, O, S):
+    sc = S.config0()
+    g = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+    o = O.CellList(sc.points, sc.cellsize, sc.dist)
+    assert_same_celllist(g, o)
+    assert g.maxpairs() == o.maxpairs()
+
+
+def test_reference_fixture(G, O, S):
+    """test/runtests.jl: GPUCellList(points, (2h, 2h), 2h) on the Dp0.02 CSVs, then update!."""
+    pts, rho, ptype = S.load_dualsphysics_csv(os.path.join(GOLDEN, "FluidPoints_Dp0.02.csv"),
+                                              os.path.join(GOLDEN, "BoundaryPoints_Dp0.02.csv"))
+    h = 1.2 * math.sqrt(2) * 0.02
+    g = G.GPUCellList(pts, (2 * h, 2 * h), 2 * h)
+    o = O.CellList(pts, (2 * h, 2 * h), 2 * h)
+    G.update_(g)
+    o.update()
+    assert_same_celllist(g, o)
+
+
+def test_config1_update_and_partialupdate(G, O, S):
+    """200,000 uniform points, cellsize = dist = 0.016, jittered by N(0, 0.001^2) per step."""
+    pts = S.config1_points()
+    g = G.GPUCellList(pts, (0.016, 0.016), 0.016)
+    o = O.CellList(pts, (0.016, 0.016), 0.016)
+    assert_same_celllist(g, o)
+    cur = pts
+    for step in range(3):
+        cur = S.jitter(cur, 12345, step)
+        g.points = cur
+        o.set_points(cur)
+        if step % 2 == 0:
+            G.partialupdate_(g)
+            o.partialupdate()
+        else:
+            G.update_(g)
+            o.update()
+        assert_same_celllist(g, o)
+
+
+def test_3d_random(G, O, S):
+    pts = S.uniform_points(77, 40000, 3)
+    g = G.GPUCellList(pts, (0.05, 0.06, 0.05), 0.05)
+    o = O.CellList(pts, (0.05, 0.06, 0.05), 0.05)
+    assert_same_celllist(g, o)
+    moved = S.jitter(pts, 77, 0, sigma=0.004)
+    g.points = moved
+    o.set_points(moved)
+    G.update_(g)
+    o.update()
+    assert_same_celllist(g, o)
+
+
+def test_float32(G, O, S):
+    pts = S.uniform_points(21, 30000, 2)
+    g = G.GPUCellList(pts, (0.011, 0.011), 0.011, precision="f32")
+    o = O.CellList(pts, (0.011, 0.011), 0.011, prec=32)
+    assert_same_celllist(g, o)
+    moved = S.jitter(pts, 21, 0, sigma=0.002)
+    g.points = moved
+    o.set_points(moved)
+    G.update_(g)
+    o.update()
+    assert_same_celllist(g, o)
+
+
+def test_small_mpairs_overflow_and_regrow(G, O, S):
+    pts = S.uniform_points(5, 5000, 2)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        g = G.GPUCellList(pts, 0.03, 0.03, mpairs=1000)
+        assert any("not full" in str(x.message) for x in w)
+    o = O.CellList(pts, 0.03, 0.03, mpairs=1000)
+    assert g.pairsn == o.pairsn > g.capacity == 1000
+    with pytest.raises(IndexError):
+        G.neighborlistview(g)
+    G.update_(g)
+    o.update()
+    assert_same_celllist(g, o)
+
+
+def test_partialupdate_overflow_keeps_true_count(G, O, S):
+    pts = S.uniform_points(6, 4000, 2)
+    g = G.GPUCellList(pts, 0.04, 0.04)
+    o = O.CellList(pts, 0.04, 0.04)
+    squeezed = tuple(0.5 + 0.6 * (p - 0.5) for p in pts)  # denser: more pairs, stale cells
+    g.points = squeezed
+    o.set_points(squeezed)
+    G.partialupdate_(g)
+    o.partialupdate()
+    assert g.pairsn == o.pairsn > g.capacity == o.info()["capacity"]
+    with pytest.raises(IndexError):
+        G.neighborlistview(g)
+    assert len(G.neighborlist(g)) == g.capacity
+
+
+def test_points_outside_min_max_clamp(G, O, S):
+    pts = S.uniform_points(8, 20000, 2)
+    kw = dict(MIN=(0.25, 0.2), MAX=(0.7, 0.9))
+    g = G.GPUCellList(pts, 0.02, 0.02, **kw)
+    o = O.CellList(pts, 0.02, 0.02, **kw)
+    assert_same_celllist(g, o)
+
+
+def test_cellsize_below_dist_warns(G, S):
+    pts = S.uniform_points(3, 1000, 2)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        g = G.GPUCellList(pts, (0.01, 0.05), 0.03)
+        assert any("Cell size" in str(x.message) for x in w)
+    assert g.cs == (0.03, 0.05)
+
+
+def test_degenerate_inputs(G, S):
+    g = G.GPUCellList((np.array([0.5]), np.array([0.5])), 0.1, 0.1, MIN=(0.0, 0.0), MAX=(1.0, 1.0))
+    assert g.pairsn == 0 and len(G.neighborlistview(g)) == 0
+    # identical points: the lowered MIN (src/celllist.jl:71) still gives one cell; all pairs at r = 0
+    g = G.GPUCellList((np.full(10, 0.3), np.full(10, 0.3)), 0.1, 0.1)
+    assert g.grid == (1, 1) and g.pairsn == 45
+    with pytest.raises(ValueError, match="no cells"):
+        G.GPUCellList((np.full(10, 0.3), np.full(10, 0.3)), 0.1, 0.1, MIN=(0.5, 0.5), MAX=(0.2, 0.2))
+    # every point in one cell: all pairs within dist
+    pts = (0.5 + 1e-3 * S.uniform01(1, 50, 0), 0.5 + 1e-3 * S.uniform01(1, 50, 1))
+    g = G.GPUCellList(pts, 0.1, 0.1)
+    assert g.pairsn == 50 * 49 // 2
+
+
+def test_deterministic_order(G, S):
+    pts = S.uniform_points(31, 50000, 2)
+    a = G.neighborlistview(G.GPUCellList(pts, 0.01, 0.01))
+    b = G.neighborlistview(G.GPUCellList(pts, 0.01, 0.01))
+    assert np.array_equal(a, b)

@@ -1,0 +1,223 @@
+"""GPU SPH step vs the CPU oracle.
+
+Tolerances.  FP64 path: per-pair terms are evaluated in the reference's
+operation order without FMA contraction, so the only difference from the
+oracle is the order of the per-particle sums (the reference's own FP64 atomic
+order is nondeterministic); fields agree to ~1e-13 relative after tens of
+steps.  FP32-split path: pair sums in FP32 (Kahan), density as ρ-ρ0, FP64
+positions; bounds follow the acceptance target of 1e-5 relative density and
+1e-4·dx positions after 1000 steps."""
+import math
+import os
+
+import numpy as np
+import pytest
+from cases import golden_cases
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("x", "v", "rho", "P", "acc", "drho", "sumW", "sumgradW")
+
+
+def gpu_fields(G, prob):
+    return {"x": np.array(G.get_points(prob)), "v": np.array(G.get_velocity(prob)), "rho": G.get_density(prob),
+            "P": G.get_pressure(prob), "acc": np.array(G.get_acceleration(prob)), "drho": G.get_drho(prob),
+            "sumW": G.get_sumw(prob), "sumgradW": np.array(G.get_sumgradw(prob))}
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+# FP64 tolerances (relative to the field's max magnitude)
+TOL64 = {"x": 1e-13, "v": 1e-10, "rho": 1e-13, "P": 1e-10, "acc": 1e-9, "drho": 1e-8, "sumW": 1e-12,
+         "sumgradW": 1e-9}
+
+
+@pytest.mark.parametrize("name", sorted(golden_cases()))
+def test_f64_matches_golden(G, name):
+    sc, steps, ts = golden_cases()[name]
+    ref = np.load(os.path.join(GOLDEN, f"oracle_{name}.npz"))
+    cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+    assert np.array_equal(np.unique(G.neighborlistview(cl), axis=0), ref["pairs0"])
+    prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, v=sc.v, precision="f64")
+    upd, mx = G.stepsolve_(prob, steps, timestepping=ts)
+    assert upd == ref["updaten"]
+    assert mx == pytest.approx(float(ref["maxcdx"]), rel=1e-12)
+    assert prob.dt == pytest.approx(float(ref["dt"]), rel=1e-12)
+    assert prob.etime == pytest.approx(float(ref["etime"]), rel=1e-12)
+    got = gpu_fields(G, prob)
+    for f in FIELDS:
+        assert rel_err(got[f], ref[f]) < TOL64[f], (f, rel_err(got[f], ref[f]))
+    assert np.array_equal(np.unique(G.neighborlistview(cl), axis=0), ref["pairs_end"])
+
+
+@pytest.mark.parametrize("name", sorted(golden_cases()))
+def test_f32split_matches_golden(G, name):
+    sc, steps, ts = golden_cases()[name]
+    ref = np.load(os.path.join(GOLDEN, f"oracle_{name}.npz"))
+    cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+    prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, v=sc.v, precision="f32split")
+    upd, _ = G.stepsolve_(prob, steps, timestepping=ts)
+    assert upd == ref["updaten"]
+    got = gpu_fields(G, prob)
+    dx = sc.params.dx
+    assert np.abs(got["x"] - ref["x"]).max() < 1e-5 * dx
+    assert np.abs(got["rho"] / ref["rho"] - 1).max() < 1e-6
+    assert rel_err(got["v"], ref["v"]) < 1e-4
+
+
+def _pair(G, O, sc, precision="f64"):
+    cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+    oc = O.CellList(sc.points, sc.cellsize, sc.dist)
+    prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, v=sc.v, precision=precision)
+    ref = O.SPHProblem(oc, sc.params, sc.rho, sc.ptype, v=sc.v)
+    return cl, oc, prob, ref
+
+
+def test_config0_f64_50_steps(G, O, S):
+    """The 2D dam break at dx=0.02 (6,491 particles), reference physics."""
+    sc = S.config0()
+    cl, oc, prob, ref = _pair(G, O, sc)
+    for n in (1, 9, 40):
+        assert G.stepsolve_(prob, n) == pytest.approx(ref.stepsolve(n), rel=1e-12)
+    assert np.abs(np.array(G.get_points(prob)) - ref.get("x")).max() < 1e-12 * sc.params.dx
+    assert np.abs(G.get_density(prob) / ref.get("rho") - 1).max() < 1e-13
+    assert rel_err(np.array(G.get_velocity(prob)), ref.get("v")) < 1e-11
+    assert rel_err(np.array(G.get_acceleration(prob)), ref.get("acc")) < 1e-9
+    assert rel_err(G.get_pressure(prob), ref.get("P")) < 1e-10
+    assert np.array_equal(np.unique(G.neighborlistview(cl), axis=0), np.unique(oc.neighborlistview(), axis=0))
+
+
+def test_config0_f32split_50_steps(G, O, S):
+    sc = S.config0()
+    cl, oc, prob, ref = _pair(G, O, sc, "f32split")
+    for n in (10, 40):
+        assert G.stepsolve_(prob, n)[0] == ref.stepsolve(n)[0]
+    assert np.abs(np.array(G.get_points(prob)) - ref.get("x")).max() < 1e-5 * sc.params.dx
+    assert np.abs(G.get_density(prob) / ref.get("rho") - 1).max() < 1e-6
+
+
+def test_reference_fixture_with_timestepping(G, O, S):
+    """The reference's Dp0.02 fixture (ptype 0/1 walls) with adaptive Δt and the examples' dist = 1.1H."""
+    pts, rho, ptype = S.load_dualsphysics_csv(os.path.join(GOLDEN, "FluidPoints_Dp0.02.csv"),
+                                              os.path.join(GOLDEN, "BoundaryPoints_Dp0.02.csv"))
+    p, dist = S.standard_params(0.02, 2, dist_factor=1.1)
+    sc = S.Scene(2, pts, rho, ptype, p, dist, (dist, dist))
+    cl, oc, prob, ref = _pair(G, O, sc)
+    r1 = G.stepsolve_(prob, 30, timestepping=True)
+    r2 = ref.stepsolve(30, True)
+    assert r1[0] == r2[0] and r1[1] == pytest.approx(r2[1], rel=1e-10)
+    assert prob.dt == pytest.approx(ref.dt, rel=1e-10)
+    assert np.abs(np.array(G.get_points(prob)) - ref.get("x")).max() < 1e-12 * p.dx
+    assert np.abs(G.get_density(prob) / ref.get("rho") - 1).max() < 1e-12
+    assert np.array_equal(np.unique(G.neighborlistview(cl), axis=0), np.unique(oc.neighborlistview(), axis=0))
+    assert cl.pairsn == oc.pairsn
+
+
+def test_3d_dam_break_all_physics(G, O, S):
+    """3D dam break (the config with viscosity, surface tension, DPC, XSPH) at a coarse spacing."""
+    sc = S.config2(dx=0.04)
+    sc.params.cspmn = 2
+    cl, oc, prob, ref = _pair(G, O, sc)
+    assert G.stepsolve_(prob, 3, timestepping=True)[0] == ref.stepsolve(3, True)[0]
+    assert np.abs(np.array(G.get_points(prob)) - ref.get("x")).max() < 1e-12 * sc.params.dx
+    assert np.abs(G.get_density(prob) / ref.get("rho") - 1).max() < 1e-12
+    assert rel_err(np.array(G.get_velocity(prob)), ref.get("v")) < 1e-10
+
+
+def test_deterministic(G, S):
+    sc = S.config0()
+    out = []
+    for _ in range(2):
+        cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, precision="f32split")
+        G.stepsolve_(prob, 20, timestepping=True)
+        out.append((np.array(G.get_points(prob)), G.get_density(prob), np.array(G.get_velocity(prob))))
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_user_update_and_set_points_between_calls(G, O, S):
+    """update!(prob.system), partialupdate! and edits of prob.system.points between stepsolve! calls."""
+    sc = golden_cases()["dam2d_allphys"][0]
+    cl, oc, prob, ref = _pair(G, O, sc)
+    G.stepsolve_(prob, 3, timestepping=True)
+    ref.stepsolve(3, True)
+    G.update_(cl)
+    oc.update()
+    assert cl.pairsn == oc.pairsn
+    assert np.array_equal(np.unique(G.neighborlistview(cl), axis=0), np.unique(oc.neighborlistview(), axis=0))
+    x = np.array(G.get_points(prob))
+    x[0] += 1e-4
+    cl.points = x
+    oc.set_points(x)
+    G.partialupdate_(cl)
+    oc.partialupdate()
+    assert np.array_equal(np.unique(G.neighborlistview(cl), axis=0), np.unique(oc.neighborlistview(), axis=0))
+    G.stepsolve_(prob, 3, timestepping=True)
+    ref.stepsolve(3, True)
+    assert np.abs(np.array(G.get_points(prob)) - ref.get("x")).max() < 1e-12 * sc.params.dx
+    assert np.abs(G.get_density(prob) / ref.get("rho") - 1).max() < 1e-12
+
+
+def test_timesolve_batches(G, O, S):
+    sc = S.dam_break_2d(0.05, 0.5, 1.0, 2.0, 1.5)
+    cl, oc, prob, ref = _pair(G, O, sc)
+    G.timesolve_(prob, batch=7, timeframe=2e-4, verbose=False)
+    while ref.etime <= 2e-4:
+        ref.stepsolve(7)
+    assert prob.etime == pytest.approx(ref.etime, rel=1e-12)
+    assert np.abs(np.array(G.get_points(prob)) - ref.get("x")).max() < 1e-12 * sc.params.dx
+
+
+def test_gamma_other_than_7(G, O, S):
+    sc = S.dam_break_2d(0.05, 0.5, 1.0, 2.0, 1.5)
+    sc.params.gamma = 5
+    cl, oc, prob, ref = _pair(G, O, sc)
+    G.stepsolve_(prob, 3)
+    ref.stepsolve(3)
+    assert np.abs(G.get_density(prob) / ref.get("rho") - 1).max() < 1e-12
+    with pytest.raises(NotImplementedError):
+        cl2 = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        G.SPHProblem.from_params(cl2, sc.params, sc.rho, sc.ptype, precision="f32split")
+
+
+def test_argument_errors(G, S):
+    sc = S.dam_break_2d(0.05, 0.5, 1.0, 2.0, 1.5)
+    cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+    with pytest.raises(ValueError, match="length"):
+        G.SPHProblem.from_params(cl, sc.params, sc.rho[:-1], sc.ptype)
+    cl32 = G.GPUCellList(sc.points, sc.cellsize, sc.dist, precision="f32")
+    with pytest.raises(NotImplementedError):
+        G.SPHProblem.from_params(cl32, sc.params, sc.rho, sc.ptype)
+
+
+def test_ptype_getter_and_stepsolve_zero(G, O, S):
+    sc = S.dam_break_2d(0.05, 0.5, 1.0, 2.0, 1.5)
+    cl, oc, prob, ref = _pair(G, O, sc)
+    assert np.array_equal(G.get_ptype(prob), sc.ptype)
+    assert G.stepsolve_(prob, 0) == (0, 0.0)
+    ref.stepsolve(0)
+    assert rel_err(G.get_sumw(prob), ref.get("sumW")) < 1e-12
+
+
+def test_full_size_f32split_vs_f64_1000_steps(G, S):
+    """The 4,047,991-particle 2D dam break: after 1000 adaptive steps the
+    FP32-split state stays within 1e-5 relative density and 1e-4·dx positions
+    of the FP64 state (the FP64 path being pinned to the oracle above)."""
+    sc = S.config3()
+    res = {}
+    for prec in ("f64", "f32split"):
+        cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, precision=prec)
+        G.stepsolve_(prob, 1000, timestepping=True)
+        res[prec] = (np.array(G.get_points(prob)), G.get_density(prob), prob.etime)
+        del prob
+    x64, r64, t64 = res["f64"]
+    x32, r32, t32 = res["f32split"]
+    assert np.abs(r32 / r64 - 1).max() < 1e-5
+    assert np.abs(x32 - x64).max() < 1e-4 * sc.params.dx
+    assert t32 == pytest.approx(t64, rel=1e-6)

@@ -1,0 +1,418 @@
+"""Pins for the CPU oracle, each independent of the oracle's own code: closed
+forms, quadrature, brute force, and physical invariants that a dropped term, a
+wrong sign, a wrong index or a wrong constant would break."""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+# --------------------------------------------------------------------------
+# WendlandC2 (SPHKernels.jl) — normalisation fixes σ, the derivative fixes dW
+# --------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_wendland_integrates_to_one(O, dim):
+    H = 0.0371
+    hinv = 1.0 / H
+    if dim == 2:
+        val, _ = integrate.quad(lambda r: O.wendland_W(2, r * hinv, hinv) * 2 * math.pi * r, 0, H, epsabs=0,
+                                epsrel=1e-13)
+    else:
+        val, _ = integrate.quad(lambda r: O.wendland_W(3, r * hinv, hinv) * 4 * math.pi * r * r, 0, H, epsabs=0,
+                                epsrel=1e-13)
+    assert abs(val - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_wendland_derivative_is_dW_dr(O, dim):
+    H = 0.05
+    hinv = 1.0 / H
+    for u in (0.05, 0.2, 0.5, 0.8, 0.97):
+        eps = 1e-6
+        fd = (O.wendland_W(dim, u + eps, hinv) - O.wendland_W(dim, u - eps, hinv)) / (2 * eps) * hinv
+        an = O.wendland_dW(dim, u, hinv)
+        assert an < 0
+        assert abs(fd - an) <= 1e-7 * abs(an)
+    assert O.wendland_W(dim, 1.0, hinv) == 0.0 and O.wendland_dW(dim, 1.2, hinv) == 0.0
+
+
+def test_wendland_support_and_peak(O):
+    # compact support (u >= 1 gives 0) and the kernel is maximal at the origin
+    for dim in (2, 3):
+        us = np.linspace(0, 0.999, 50)
+        w = [O.wendland_W(dim, u, 2.0) for u in us]
+        assert all(a > b for a, b in zip(w, w[1:]))
+
+
+# --------------------------------------------------------------------------
+# powfancy7th (src/gpukernels_sph.jl:499-515) and the DPC threshold
+# --------------------------------------------------------------------------
+
+
+def test_powfancy7th_accuracy(O):
+    xs = np.linspace(0.98, 1.02, 2001)
+    err = max(abs(O.powfancy7th(x) / x ** (1.0 / 7.0) - 1.0) for x in xs)
+    assert err < 1e-12  # 2 Newton steps from the magic seed
+    # the 2-step iteration is not exact at 1 (a known property of the method)
+    assert abs(O.powfancy7th(1.0) - 1.0) < 1e-12
+    for x in (0.5, 2.0, 1e-3, 30.0):
+        assert abs(O.powfancy7th(x) / x ** (1 / 7) - 1) < 1e-4
+
+
+@pytest.mark.parametrize("dim,lo,hi", [(2, 0.30, 0.40), (3, 0.38, 0.46)])
+def test_dpckernlim_root(O, dim, lo, hi):
+    r = O.dpckernlim(dim)
+    assert lo < r < hi
+    assert abs(1.0 - O.wendland_W(dim, r, 1.0)) < 1e-14
+
+
+# --------------------------------------------------------------------------
+# Tait equation of state (src/gpukernels_sph.jl:691-697)
+# --------------------------------------------------------------------------
+
+
+def test_eos_closed_form(O):
+    rho0, c0 = 1000.0, 20 * math.sqrt(2 * 9.81)
+    B = Fraction(c0) ** 2 * Fraction(rho0) / 7
+    for rho in (1000.0, 1000.5, 1003.25, 1040.0, 999.0):
+        exact = float(B * ((Fraction(rho) / Fraction(rho0)) ** 7 - 1))
+        assert O.eos(rho, 7, rho0, c0, 2) == pytest.approx(exact, rel=1e-13, abs=1e-9)
+    assert O.eos(999.0, 7, rho0, c0, 1) == 0.0  # boundary below ρ0 clamps to 0
+    assert O.eos(999.0, 7, rho0, c0, -1) == 0.0
+    assert O.eos(999.0, 7, rho0, c0, 2) < 0.0  # fluid may go to tension
+
+
+# --------------------------------------------------------------------------
+# Cell list against brute force
+# --------------------------------------------------------------------------
+
+
+def brute_pairs(points, dist, cells=None):
+    """All i<j with the reference's r^2 expression < dist^2 and (optionally)
+    cell coordinates differing by at most one along every axis."""
+    D = len(points)
+    n = len(points[0])
+    r2 = np.zeros((n, n))
+    for p in points:
+        d = p[:, None] - p[None, :]
+        r2 = r2 + d * d
+    ok = r2 < dist * dist
+    if cells is not None:
+        for c in cells:
+            ok &= np.abs(c[:, None].astype(np.int64) - c[None, :]) <= 1
+    i, j = np.nonzero(np.triu(ok, 1))
+    return set(zip((i + 1).tolist(), (j + 1).tolist()))
+
+
+def closed_form_cells(points, info):
+    out = []
+    for k, p in enumerate(points):
+        hinv = 1.0 / info["cs"][k]
+        q = (p - info["offset"][k]) * hinv
+        out.append(np.ceil(np.minimum(np.maximum(q, 1.0), info["grid"][k])).astype(np.int32))
+    return out
+
+
+@pytest.mark.parametrize("dim,n,dist,csf,seed", [(2, 700, 0.07, 1.0, 1), (2, 500, 0.05, 1.7, 2), (3, 600, 0.16, 1.0, 3),
+                                                 (3, 400, 0.2, 1.3, 4)])
+def test_celllist_pairs_bruteforce(O, S, dim, n, dist, csf, seed):
+    pts = S.uniform_points(seed, n, dim)
+    cl = O.CellList(pts, dist * csf, dist)
+    inf = cl.info()
+    cells = closed_form_cells(pts, inf)
+    pc, cp = cl.cells()
+    for k in range(dim):
+        assert np.array_equal(pc[k], cells[k])
+    got = set(map(tuple, cl.neighborlistview().tolist()))
+    assert len(got) == cl.pairsn
+    assert all(i < j for i, j in got)
+    assert got == brute_pairs(pts, dist, cells)
+    assert cl.maxpairs() >= cl.pairsn
+    counts = np.bincount(np.ravel_multi_index([c - 1 for c in cells], inf["grid"], order="F"),
+                         minlength=int(np.prod(inf["grid"])))
+    assert cp.ravel(order="F").tolist() == counts.tolist()
+    assert inf["mppcell"] >= math.ceil(counts.max() * 1.05 + 1)
+
+
+def test_celllist_grid_offset(O, S):
+    pts = S.uniform_points(9, 300, 2)
+    cl = O.CellList(pts, (0.1, 0.13), 0.1)
+    inf = cl.info()
+    for k in range(2):
+        mn = pts[k].min()
+        off = mn - abs((mn + math.sqrt(np.finfo(float).eps)) * math.sqrt(np.finfo(float).eps))
+        assert inf["offset"][k] == off
+        assert inf["grid"][k] == math.ceil((pts[k].max() - off) / inf["cs"][k])
+
+
+def test_cellsize_below_dist_is_raised(O, S):
+    pts = S.uniform_points(5, 100, 2)
+    cl = O.CellList(pts, (0.02, 0.2), 0.05)
+    assert cl.warn & 1
+    assert cl.info()["cs"] == (0.05, 0.2)
+
+
+def test_partialupdate_uses_stale_cells(O, S):
+    pts = S.uniform_points(11, 800, 2)
+    dist = 0.05
+    cl = O.CellList(pts, dist, dist)
+    old_cells = closed_form_cells(pts, cl.info())
+    moved = S.jitter(pts, 11, 0, sigma=0.02)
+    cl.set_points(moved)
+    cl.partialupdate()
+    got = set(map(tuple, cl.neighborlistview().tolist()))
+    assert got == brute_pairs(moved, dist, old_cells)
+    cl.update()
+    new_cells = closed_form_cells(moved, cl.info())
+    got = set(map(tuple, cl.neighborlistview().tolist()))
+    assert got == brute_pairs(moved, dist, new_cells)
+
+
+def test_pair_capacity_rules(O, S):
+    pts = S.uniform_points(13, 1000, 2)
+    dist = 0.05
+    full = O.CellList(pts, dist, dist)
+    total = full.pairsn
+    assert full.info()["capacity"] == math.ceil(total / 0.8)  # shrink rule, src/celllist.jl:109-115
+    small = O.CellList(pts, dist, dist, mpairs=100)
+    assert small.warn & 2 and small.pairsn == total and small.info()["capacity"] == 100
+    with pytest.raises(O.OracleError):
+        small.neighborlistview()
+    small.update()  # regrow to ceil(pairsn / 0.8) (src/celllist.jl:146-148)
+    assert small.info()["capacity"] == math.ceil(total / 0.8)
+    assert set(map(tuple, small.neighborlistview().tolist())) == set(map(tuple, full.neighborlistview().tolist()))
+
+
+def test_points_outside_grid_clamp(O, S):
+    pts = S.uniform_points(17, 400, 2)
+    cl = O.CellList(pts, 0.1, 0.1, MIN=(0.2, 0.2), MAX=(0.8, 0.8))
+    inf = cl.info()
+    pc, _ = cl.cells()
+    assert pc.min() == 1 and pc[0].max() == inf["grid"][0] and pc[1].max() == inf["grid"][1]
+    cells = closed_form_cells(pts, inf)
+    assert set(map(tuple, cl.neighborlistview().tolist())) == brute_pairs(pts, 0.1, cells)
+
+
+def test_float32_celllist(O, S):
+    pts = S.uniform_points(19, 600, 2)
+    cl = O.CellList(pts, 0.06, 0.06, prec=32)
+    p32 = [p.astype(np.float32) for p in pts]
+    inf = cl.info()
+    # constructor binning uses the Float64 offset with Float32 points (src/celllist.jl:84)
+    cells = []
+    for k in range(2):
+        hinv = np.float32(1) / np.float32(inf["cs"][k])
+        mn = float(p32[k].min())
+        off64 = mn - abs((mn + math.sqrt(np.finfo(float).eps)) * math.sqrt(np.finfo(float).eps))
+        q = (p32[k].astype(np.float64) - off64) * np.float64(hinv)
+        cells.append(np.ceil(np.minimum(np.maximum(q, 1.0), inf["grid"][k])).astype(np.int32))
+    d = np.float32(0.06)
+    r2 = np.zeros((600, 600), np.float32)
+    for p in p32:
+        dd = p[:, None] - p[None, :]
+        r2 = r2 + dd * dd
+    ok = r2 < d * d
+    for c in cells:
+        ok &= np.abs(c[:, None].astype(np.int64) - c[None, :]) <= 1
+    i, j = np.nonzero(np.triu(ok, 1))
+    assert set(map(tuple, cl.neighborlistview().tolist())) == set(zip((i + 1).tolist(), (j + 1).tolist()))
+
+
+def test_reference_fixture_celllist(O, S):
+    """test/runtests.jl:8-16: GPUCellList(points, (2h, 2h), 2h) on the Dp0.02 CSVs, then update!."""
+    import os
+
+    from conftest import GOLDEN
+
+    pts, rho, ptype = S.load_dualsphysics_csv(os.path.join(GOLDEN, "FluidPoints_Dp0.02.csv"),
+                                              os.path.join(GOLDEN, "BoundaryPoints_Dp0.02.csv"))
+    assert len(pts[0]) == 4704 + 1491
+    dx = 0.02
+    h = 1.2 * math.sqrt(2) * dx
+    cl = O.CellList(pts, (2 * h, 2 * h), 2 * h)
+    cl.update()
+    cells = closed_form_cells(pts, cl.info())
+    assert set(map(tuple, cl.neighborlistview().tolist())) == brute_pairs(pts, 2 * h, cells)
+
+
+# --------------------------------------------------------------------------
+# SPH step invariants
+# --------------------------------------------------------------------------
+
+
+def _problem(O, sc, **over):
+    for k, v in over.items():
+        setattr(sc.params, k, v)
+    cl = O.CellList(sc.points, sc.cellsize, sc.dist)
+    return O.SPHProblem(cl, sc.params, sc.rho, sc.ptype, v=sc.v)
+
+
+def test_hydrostatic_column_is_in_balance(O, S):
+    """The reference fixture at rest: inner fluid particles feel ~0 net force
+    (pressure gradient of the Tait-hydrostatic density balances gravity)."""
+    import os
+
+    from conftest import GOLDEN
+
+    pts, rho, ptype = S.load_dualsphysics_csv(os.path.join(GOLDEN, "FluidPoints_Dp0.02.csv"),
+                                              os.path.join(GOLDEN, "BoundaryPoints_Dp0.02.csv"))
+    p, dist = S.standard_params(0.02, 2)
+    x, y = pts
+    inner = (ptype == 2) & (x > 0.3) & (x < 0.75) & (y > 0.3) & (y < 1.7)
+    assert inner.sum() > 500
+    # the fixture's own Rhop is rounded to 0.1 kg/m^3: a staircase that adds
+    # particle-scale noise, but the mean force still balances gravity
+    sph = _problem(O, S.Scene(2, pts, rho, ptype, p, dist, (p.H, p.H)))
+    sph.stepsolve(1)
+    a = sph.get("acc")
+    assert abs(a[1][inner].mean()) < 0.05 * 9.81
+    assert abs(a[0][inner].mean()) < 0.01 * 9.81
+    # the exact Tait-hydrostatic density on the same particles balances pointwise
+    exact = np.where(ptype == 2, S.hydrostatic_rho(y, 2.0, p.rho0, p.g, p.c0), rho)
+    sph = _problem(O, S.Scene(2, pts, exact, ptype, p, dist, (p.H, p.H)))
+    sph.stepsolve(1)
+    a = sph.get("acc")
+    assert np.abs(a[1][inner]).max() < 0.02 * 9.81
+    assert np.abs(a[0][inner]).max() < 0.01 * 9.81
+
+
+def _lattice(S, n=20, dx=0.02, seed=3, vscale=0.3):
+    p, dist = S.standard_params(dx, 2)
+    g = np.arange(n) * dx
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    x = X.ravel() + 0.1 * dx * (S.uniform01(seed, n * n, 0) - 0.5)
+    y = Y.ravel() + 0.1 * dx * (S.uniform01(seed, n * n, 1) - 0.5)
+    v = (vscale * (S.uniform01(seed, n * n, 2) - 0.5), vscale * (S.uniform01(seed, n * n, 3) - 0.5))
+    rho = 1000.0 + 2.0 * S.uniform01(seed, n * n, 4)
+    ptype = np.full(n * n, 2, np.int32)
+    sc = S.Scene(2, (x, y), rho, ptype, p, dist, (p.H, p.H), v=v)
+    return sc
+
+
+def test_momentum_conservation(O, S):
+    """Every pair term is applied as +F to one side and -F to the other, so the
+    total acceleration is the gravity of every particle."""
+    sc = _lattice(S)
+    sph = _problem(O, sc, nu=0.05, s=0.01)
+    sph.stepsolve(1)
+    a = sph.get("acc")
+    n = sc.n
+    scale = np.abs(a).sum()
+    assert abs(a[0].sum()) < 1e-12 * scale
+    assert abs(a[1].sum() + n * 9.81) < 1e-12 * scale
+
+
+def test_galilean_invariance(O, S):
+    sc = _lattice(S)
+    ref = _problem(O, sc)
+    ref.stepsolve(1)
+    sc2 = _lattice(S)
+    sc2.v = (sc2.v[0] + 0.7, sc2.v[1] - 0.4)
+    mov = _problem(O, sc2)
+    mov.stepsolve(1)
+    assert np.allclose(mov.get("drho"), ref.get("drho"), rtol=1e-9, atol=1e-9 * np.abs(ref.get("drho")).max())
+    assert np.allclose(mov.get("acc"), ref.get("acc"), rtol=1e-9, atol=1e-9 * np.abs(ref.get("acc")).max())
+
+
+def test_isolated_particle_update_closed_form(O, S):
+    """A particle without partners: symplectic update and Δt stepping in closed form."""
+    p, dist = S.standard_params(0.02, 2, dt=2e-5)
+    x0, y0, vx, vy = 0.3, 0.5, 0.2, -0.1
+    sc = S.Scene(2, (np.array([x0, x0 + 10.0]), np.array([y0, y0])), np.array([1001.0, 1000.0]),
+                 np.array([2, 2], np.int32), p, dist, (p.H, p.H), v=(np.array([vx, 0.0]), np.array([vy, 0.0])))
+    sph = _problem(O, sc)
+    dt = p.dt
+    tl = (1e-9, 1.0)
+    sph.stepsolve(1, True, tl)
+    x = sph.get("x")
+    v = sph.get("v")
+    vy1 = vy - 9.81 * dt
+    assert v[0][0] == pytest.approx(vx, abs=1e-15) and v[1][0] == pytest.approx(vy1, rel=1e-14)
+    assert x[0][0] == pytest.approx(x0 + vx * dt, rel=1e-14)
+    assert x[1][0] == pytest.approx(y0 + (vy + vy1) / 2 * dt, rel=1e-14)
+    assert sph.get("rho")[0] == 1001.0
+    # Δt: dt1 = sqrt(h / 3 max|a|) with |a| = g; visc uses absolute positions
+    h = p.h
+    dt1 = math.sqrt(h / (3 * 9.81))
+    visc = max(abs(h * (vx * x[0][0] + vy1 * x[1][0]) / (x[0][0] ** 2 + x[1][0] ** 2 + (0.01 * h) ** 2)),
+               abs(h * (0 * x[0][1] + (-9.81 * dt) * x[1][1]) / (x[0][1] ** 2 + x[1][1] ** 2 + (0.01 * h) ** 2)))
+    dt2 = h / (p.c0 + visc)
+    assert sph.dt == pytest.approx(min(max(p.CFL * min(dt1, dt2), tl[0]), tl[1]), rel=1e-12)
+
+
+def test_pair_force_signs(O, S):
+    """Two fluid particles: pressure pushes them apart, artificial viscosity
+    opposes their approach, and the density rises while they approach."""
+    p, dist = S.standard_params(0.02, 2)
+    p.g = 0.0
+    d = 0.6 * p.H
+    pts = (np.array([0.5, 0.5 + d]), np.array([0.5, 0.5]))
+    sc = S.Scene(2, pts, np.array([1002.0, 1002.0]), np.array([2, 2], np.int32), p, dist, (p.H, p.H),
+                 v=(np.array([0.5, -0.5]), np.array([0.0, 0.0])))
+    sph = _problem(O, sc, cspmn=0)
+    sph.stepsolve(1)
+    a = sph.get("acc")
+    assert a[0][0] < 0 < a[0][1] and a[0][0] == pytest.approx(-a[0][1], rel=1e-12)
+    assert sph.get("drho")[0] > 0 and sph.get("drho")[1] > 0
+    # same geometry at ρ0 (no pressure): only artificial viscosity acts
+    sc.rho = np.array([1000.0, 1000.0])
+    sph = _problem(O, sc, cspmn=0)
+    sph.stepsolve(1)
+    a = sph.get("acc")
+    assert a[0][0] < 0 < a[0][1]
+
+
+def test_boundary_force_repels(O, S):
+    p, dist = S.standard_params(0.02, 2)
+    p.g = 0.0
+    pts = (np.array([0.5, 0.5 + 1.5 * 0.02]), np.array([0.5, 0.5]))
+    sc = S.Scene(2, pts, np.array([1000.0, 1000.0]), np.array([2, -1], np.int32), p, dist, (p.H, p.H))
+    sph = _problem(O, sc, cspmn=0)
+    sph.stepsolve(1)
+    assert sph.get("acc")[0][0] < 0  # fluid pushed away from the wall particle
+
+
+def test_cspm_reproduces_uniform_density(O, S):
+    sc = _lattice(S, vscale=0.0)
+    sc.rho = np.full(sc.n, 1003.0)
+    sph = _problem(O, sc, cspmn=1, g=0.0)
+    sph.stepsolve(1)
+    rho = sph.get("rho")
+    # renormalisation of a uniform field returns it (up to the step's own change)
+    sph2 = _problem(O, _lattice(S, vscale=0.0), cspmn=0, g=0.0)
+    sc3 = _lattice(S, vscale=0.0)
+    sc3.rho = np.full(sc3.n, 1003.0)
+    ref = _problem(O, sc3, cspmn=0, g=0.0)
+    ref.stepsolve(1)
+    assert np.abs(rho / ref.get("rho") - 1).max() < 2e-3
+    del sph2
+
+
+def test_xsph_and_dpc_change_positions_only_of_fluid(O, S):
+    sc = _lattice(S)
+    sc.ptype[:40] = 1
+    base = _problem(O, sc, cspmn=0)
+    base.stepsolve(1)
+    sc2 = _lattice(S)
+    sc2.ptype[:40] = 1
+    corr = _problem(O, sc2, cspmn=0, xsph_eps=0.5, dpc_l0=0.01, dpc_lambda=0.01)
+    corr.stepsolve(1)
+    dxa = np.abs(corr.get("x") - base.get("x")).max(axis=0)
+    assert dxa[:40].max() == 0.0  # ptype 1 particles are not corrected
+    assert dxa[40:].max() > 0.0
+
+
+def test_rebuild_policy(O, S):
+    sc = _lattice(S)
+    sph = _problem(O, sc)
+    upd, mx = sph.stepsolve(5)
+    assert upd == 5 and mx > 0  # dist == H: nui = 0, every step rebuilds
+    sc = _lattice(S)
+    sc.params.nui = 0.1 * sc.params.H
+    sc.dist = 1.1 * sc.params.H
+    sph = _problem(O, sc)
+    upd, mx = sph.stepsolve(5)
+    assert 1 <= upd < 5
