@@ -106,7 +106,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pass2_f32split_traffic.txt"))
+    ap.add_argument("--calibrate-copy", type=float, default=0.0,
+                    help="copy a buffer of this many GiB three times and exit (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy > 0:
+        import torch
+
+        n = int(args.calibrate_copy * (1 << 30)) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: the SPH step is built for a single MI355X "
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)

@@ -270,7 +270,7 @@ struct sph_s {
     B.PH = dalloc<R>(n);
     B.TMP = dalloc<double>(n);
     B.nblk = int(grid_for(n));
-    B.RED = dalloc<double>(3 * int64_t(B.nblk));
+    B.RED = dalloc<double>(3 * 64);  // zeroed here; k_finalize clears it after every step
     B.SUMW = dalloc<double>(n);
     B.start = sb.start;
     B.skey = sb.skey;

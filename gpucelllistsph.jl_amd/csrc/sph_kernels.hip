@@ -234,10 +234,13 @@ __device__ __forceinline__ void for_each_neighbor(const Rows<D>& rows, int64_t a
 
 __device__ __forceinline__ double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
 
-// Block maximum (NaN wins) written to out[blockIdx.x]; k_finalize reduces the
-// per-block values, so no same-address atomics are needed.  Every thread of
-// the block must call it.
-__device__ __forceinline__ void block_max(double v, bool active, double* out) {
+constexpr int kRedSlots = 64;
+
+// Block maximum (NaN wins) folded into one of kRedSlots slots by an atomic max
+// on its bit pattern (non-negative doubles order like their bits, NaN above
+// +Inf); spreading blocks over many slots avoids one hot address.  Every
+// thread of the block must call it.
+__device__ __forceinline__ void block_max(double v, bool active, double* slots) {
   __shared__ double wm[kBlock / 64];
   double m = wave_max(active ? v : 0.0);
   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
@@ -246,9 +249,22 @@ __device__ __forceinline__ void block_max(double v, bool active, double* out) {
     double r = wm[0];
 #pragma unroll
     for (int w = 1; w < kBlock / 64; ++w) r = nanmax(r, wm[w]);
-    out[blockIdx.x] = r;
+    atomic_max_nonneg(reinterpret_cast<unsigned long long*>(slots) + (blockIdx.x % kRedSlots), r);
   }
   __syncthreads();
+}
+
+
+// Particle index of this thread with an XCD-aware block order.  Workgroups are
+// dealt to the 8 XCDs round robin (block b runs on XCD b % 8); renumbering so
+// that each XCD walks one contiguous range of cell-sorted particles keeps the
+// neighbourhoods shared by consecutive blocks (and by the stencil rows above
+// and below) inside one XCD's L2 instead of fetching them on every XCD.
+__device__ __forceinline__ int64_t xcd_particle_index() {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned q = G / 8, r = G % 8, xcd = b % 8, i = b / 8;
+  const unsigned L = xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
+  return int64_t(L) * blockDim.x + threadIdx.x;
 }
 
 // ------------------------------------------------------------------ kernels
@@ -306,7 +322,7 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const double* X[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
   const R* V[3] = {B.V[p][0], B.V[p][1], B.V[p][D == 3 ? 2 : 1]};
@@ -429,6 +445,7 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
 #pragma clang fp contract(fast)
     const float dra = sa;
     const float rhoa = C.f_rho0 + dra;
+    const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
     const float Pa = B.PN[a];
     for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
       const int32_t ptb = PT[b];
@@ -439,13 +456,16 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
         dv[d] = va[d] - V[d][b];
       }
       const float r2 = dotf<D>(dx, dx);
-      const float r = sqrtf(r2);
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      const float r = r2 * rinv;
       const float u = r * C.f_Hinv;
-      const float dwk_r = dW_f(u, C.f_dwnorm) / r;
+      const float dwk_r = dW_f(u, C.f_dwnorm) * rinv;
+      const float inv_den = __builtin_amdgcn_rcpf(r2 + C.f_eta2);
 #pragma unroll
       for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
       const float drb = RHO[b];
       const float rhob = C.f_rho0 + drb;
+      const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
       const bool both = pta > 0 && ptb > 0;
       float ax[3] = {0.0f, 0.0f, 0.0f};
       if (both) {
@@ -453,33 +473,33 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
         float s = C.f_m0 * dotf<D>(dv, gw);
         if (pta > 1) {
           float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
-          float vd = C.f_DDTkh * ((drb - dra) - drhop) / (r2 + C.f_eta2);
-          s += vd * dot3 * C.f_m0 / rhob;
+          float vd = C.f_DDTkh * ((drb - dra) - drhop) * inv_den;
+          s += vd * dot3 * C.f_m0 * inv_rhob;
         }
         drho.add(s);
       }
       if ((long long)pta * (long long)ptb > 0) {
-        float Pfac = (Pa + B.PN[b]) / (rhoa * rhob);
+        float Pfac = (Pa + B.PN[b]) * (inv_rhoa * inv_rhob);
 #pragma unroll
         for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
       }
       if (both) {
         float cond = dotf<D>(dv, dx);
         if (cond < 0.0f) {
-          float dmu = C.f_h * cond / (r2 + C.f_eta2);
-          float dPi = (-C.f_alpha * C.f_c0 * dmu) / ((rhoa + rhob) * 0.5f);
+          float dmu = C.f_h * cond * inv_den;
+          float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
 #pragma unroll
           for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
         }
         if (F.nu_on) {
-          float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.f_eta2));
+          float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
 #pragma unroll
           for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
         }
         if (F.s_on && r < C.f_st_2h) {
           float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
 #pragma unroll
-          for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.f_m0;
+          for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] * (1.0f / C.f_m0);
         }
       }
       if ((pta > 0) != (ptb > 0) && r < C.f_fb_r0[0]) {
@@ -537,7 +557,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
   StepCtl* ctl = B.ctl;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   const bool active = a < B.n;
   double red_cdx = 0.0, red_acc = 0.0, red_visc = 0.0;
   if (active) {
@@ -673,6 +693,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
 #pragma clang fp contract(fast)
       const float dra = sha;
       const float rhoa = C.f_rho0 + dra;
+      const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
       Pha = B.PH[a];
       for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
         const int32_t ptb = PT[b];
@@ -684,13 +705,16 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
           dv[d] = vha[d] - VH[d][b];
         }
         const float r2n = dotf<D>(dxn, dxn);
-        const float rn = sqrtf(r2n);
-        const float dwk_r = dW_f(rn * C.f_Hinv, C.f_dwnorm) / rn;
+        const float rninv = __builtin_amdgcn_rsqf(r2n);
+        const float rn = r2n * rninv;
+        const float dwk_r = dW_f(rn * C.f_Hinv, C.f_dwnorm) * rninv;
 #pragma unroll
         for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
         const float r2 = dotf<D>(dx, dx);
+        const float inv_den = __builtin_amdgcn_rcpf(r2 + C.f_eta2);
         const float drb = RH[b];
         const float rhob = C.f_rho0 + drb;
+        const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
         const bool both = pta > 0 && ptb > 0;
         float ax[3] = {0.0f, 0.0f, 0.0f};
         if (both) {
@@ -698,26 +722,26 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
           float s = C.f_m0 * dotf<D>(dv, gw);
           if (pta > 1) {
             float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
-            float vd = C.f_DDTkh * ((drb - dra) - drhop) / (r2 + C.f_eta2);
-            s += vd * dot3 * C.f_m0 / rhob;
+            float vd = C.f_DDTkh * ((drb - dra) - drhop) * inv_den;
+            s += vd * dot3 * C.f_m0 * inv_rhob;
           }
           drho.add(s);
         }
         if ((long long)pta * (long long)ptb > 0) {
-          float Pfac = (Pha + B.PH[b]) / (rhoa * rhob);
+          float Pfac = (Pha + B.PH[b]) * (inv_rhoa * inv_rhob);
 #pragma unroll
           for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
         }
         if (both) {
           float cond = dotf<D>(dv, dx);
           if (cond < 0.0f) {
-            float dmu = C.f_h * cond / (r2 + C.f_eta2);
-            float dPi = (-C.f_alpha * C.f_c0 * dmu) / ((rhoa + rhob) * 0.5f);
+            float dmu = C.f_h * cond * inv_den;
+            float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
 #pragma unroll
             for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
           }
           if (F.nu_on) {
-            float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.f_eta2));
+            float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
 #pragma unroll
             for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
           }
@@ -726,7 +750,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
             if (r < C.f_st_2h) {
               float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
 #pragma unroll
-              for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.f_m0;
+              for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] * (1.0f / C.f_m0);
             }
           }
         }
@@ -792,8 +816,8 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
   }
   block_max(red_cdx, active, B.RED);
   if (F.ts) {
-    block_max(red_acc, active, B.RED + B.nblk);
-    if (F.visc_in_pass2) block_max(red_visc, active, B.RED + 2 * B.nblk);
+    block_max(red_acc, active, B.RED + kRedSlots);
+    if (F.visc_in_pass2) block_max(red_visc, active, B.RED + 2 * kRedSlots);
   }
 }
 
@@ -805,7 +829,7 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SphBufs<R> B, SphConst C, CellG
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const double* X[3] = {B.X[q][0], B.X[q][1], B.X[q][D == 3 ? 2 : 1]};
   const R* V[3] = {B.V[q][0], B.V[q][1], B.V[q][D == 3 ? 2 : 1]};
@@ -862,7 +886,7 @@ __global__ void k_dpc_apply(SphBufs<R> B) {
   const StepCtl* ctl = B.ctl;
   const int q = ctl->parity ^ 1, pp = ctl->pparity;
   const double dt = ctl->dt;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   if (a >= B.n || B.PT[pp][a] <= 1) return;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
@@ -879,7 +903,7 @@ __global__ __launch_bounds__(256) void k_xsph(SphBufs<R> B, SphConst C, CellGeom
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const int32_t* PT = B.PT[pp];
   const int32_t pta = PT[a];
@@ -914,7 +938,7 @@ __global__ __launch_bounds__(256) void k_cspm_sum(SphBufs<R> B, SphConst C, Cell
 #pragma clang fp contract(off)
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const int32_t* PT = B.PT[pp];
   const int32_t pta = PT[a];
@@ -943,7 +967,7 @@ template <typename R>
 __global__ void k_cspm_apply(SphBufs<R> B, SphConst C) {
   const StepCtl* ctl = B.ctl;
   const int q = ctl->parity ^ 1, pp = ctl->pparity;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   double nr = B.TMP[a];
   if (nr != nr) return;
@@ -963,7 +987,7 @@ template <typename R>
 __global__ void k_init_pn(SphBufs<R> B, SphConst C) {
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, pp = ctl->pparity;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const R rv = B.RHO[p][a];
   const int32_t pt = B.PT[pp][a];
@@ -976,7 +1000,7 @@ template <int D, typename R>
 __global__ void k_dt_visc(SphBufs<R> B, SphConst C) {
   StepCtl* ctl = B.ctl;
   const int q = ctl->parity ^ 1;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   const bool active = a < B.n;
   double v = 0.0;
   if (active) {
@@ -988,7 +1012,7 @@ __global__ void k_dt_visc(SphBufs<R> B, SphConst C) {
     }
     v = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
   }
-  block_max(v, active, B.RED + 2 * B.nblk);
+  block_max(v, active, B.RED + 2 * kRedSlots);
 }
 
 __device__ __forceinline__ double jl_min(double a, double b) { return (a != a || b != b) ? a + b : (a < b ? a : b); }
@@ -996,23 +1020,16 @@ __device__ __forceinline__ double jl_max(double a, double b) { return (a != a ||
 
 // End of step: commits the new state, rebuild flag, time and Δt
 // (src/sphproblem.jl:325-335, src/gpukernels_sph.jl:1293-1323).
-__global__ void k_finalize(StepCtl* ctl, const double* __restrict__ red, int nblk, SphConst C, int ts, int corr_on) {
-  double loc[3] = {0.0, 0.0, 0.0};
-  const int ns = ts ? 3 : 1;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x)
-    for (int k = 0; k < ns; ++k) loc[k] = nanmax(loc[k], red[k * nblk + b]);
+__global__ void k_finalize(StepCtl* ctl, double* __restrict__ red, SphConst C, int ts, int corr_on) {
+  // one wave: lane l reads slot l of each reduction and clears it for the next step
+  const int l = threadIdx.x;
   double tot[3];
   for (int k = 0; k < 3; ++k) {
-    __shared__ double wm[kBlock / 64];
-    double m = wave_max(loc[k]);
-    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
-    __syncthreads();
-    double r = wm[0];
-    for (int w = 1; w < kBlock / 64; ++w) r = nanmax(r, wm[w]);
-    tot[k] = r;
-    __syncthreads();
+    const double v = red[k * kRedSlots + l];
+    red[k * kRedSlots + l] = 0.0;
+    tot[k] = wave_max(v);
   }
-  if (threadIdx.x != 0) return;
+  if (l != 0) return;
   double cdx = tot[0];
   if (cdx > 0.9 * C.nui) ctl->rebuild = 1;
   ctl->maxcdx_out = jl_max(ctl->maxcdx_out, cdx);
@@ -1040,7 +1057,7 @@ __global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom
 #pragma clang fp contract(off)
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, pp = ctl->pparity;
-  const int64_t a = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const double* X[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
   // membership positions: 0 = stored rebuild positions, 1 = the step's start
@@ -1115,7 +1132,7 @@ void launch_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb, hi
 }
 void launch_finalize(StepCtl* ctl, const double* red, int nblk, const SphConst& C, int ts, int corr_on,
                      hipStream_t s) {
-  k_finalize<<<1, kBlock, 0, s>>>(ctl, red, nblk, C, ts, corr_on);
+  k_finalize<<<1, kRedSlots, 0, s>>>(ctl, const_cast<double*>(red), C, ts, corr_on);
 }
 
 #define GCLSPH_INSTANTIATE(D, R)                                                                               \
