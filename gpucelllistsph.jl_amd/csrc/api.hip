@@ -7,6 +7,7 @@
 // steps with no host synchronisation until the batch ends.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -198,6 +199,9 @@ struct sph_s {
   bool synced_prev = true;
   bool p_from_dpc = false;  // get_pressure reads the DPC-stage pressure
   int xrb_mode = 0;  // membership positions for diagnostics / pair list: 0 XRB, 1 other, 2 current
+  uint64_t* ne_buf = nullptr;  // neighbour-mask entries, necap per particle
+  int necap = 0;
+  bool masks_valid = false;  // NE/NC hold the membership of the last rebuild
   bool timing = false;
   double stage_ms[5] = {0, 0, 0, 0, 0};
   int64_t stage_cnt[5] = {0, 0, 0, 0, 0};
@@ -210,6 +214,7 @@ struct sph_s {
   ~sph_s() {
     for (void* p : allocs)
       if (p) (void)hipFree(p);
+    if (ne_buf) (void)hipFree(ne_buf);
     sb.release();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -229,7 +234,7 @@ struct sph_s {
     B.n = n;
     for (int w = 0; w < 2; ++w) {
       for (int k = 0; k < dim; ++k) {
-        B.X[w][k] = dalloc<double>(n);
+        B.X[w][k] = dalloc<double>(n + kPosPad);
         B.V[w][k] = dalloc<R>(n);
       }
       B.RHO[w] = dalloc<R>(n);
@@ -241,7 +246,7 @@ struct sph_s {
       B.VH[k] = dalloc<R>(n);
       B.XH2[k] = dalloc<double>(n);
       B.VH2[k] = dalloc<R>(n);
-      B.XRB[k] = dalloc<double>(n);
+      B.XRB[k] = dalloc<double>(n + kPosPad);
       B.CDX[k] = dalloc<double>(n);
       B.ACC[k] = dalloc<R>(n);
       B.DV[k] = dalloc<R>(n);
@@ -277,6 +282,34 @@ struct sph_s {
     B.perm = sb.perm;
     B.maxcell = sb.maxcell;
     B.ctl = ctl;
+    B.NC = dalloc<int32_t>(n);
+    B.NE = nullptr;
+    B.necap = 0;
+  }
+
+  // Entry store of k_nmask: a particle needs at most one entry per 32-candidate
+  // window of its 3^D stencil rows, each row holding at most 3 * mppcell
+  // candidates.  GCLSPH_NMASK_CAP lowers the capacity (particles beyond it
+  // scan their stencil instead).  Reallocation drops the stored membership.
+  template <typename R>
+  void ensure_nmask(SphBufs<R>& B) {
+    const int64_t mpp = std::max<int64_t>({hctl.mppcell, sys->mppcell, 1});
+    const int64_t rows = dim == 2 ? 3 : 9;
+    int64_t need = rows * ((3 * mpp + 31) / 32);
+    if (const char* env = std::getenv("GCLSPH_NMASK_CAP")) {
+      const long v = std::strtol(env, nullptr, 10);
+      if (v >= 0 && v < need) need = v;
+    }
+    need = std::min<int64_t>(need, 4096);
+    if (need == necap && ne_buf) return;
+    GCL_HIP(hipStreamSynchronize(stream));
+    if (ne_buf) GCL_HIP(hipFree(ne_buf));
+    ne_buf = nullptr;
+    GCL_HIP(hipMalloc(&ne_buf, sizeof(uint64_t) * size_t(std::max<int64_t>(need, 1)) * size_t(n)));
+    necap = int(need);
+    B.NE = ne_buf;
+    B.necap = necap;
+    masks_valid = false;
   }
 
   // Constants derived from the parameters, as the reference derives them
@@ -434,11 +467,15 @@ struct sph_s {
     GCL_HIP(hipGetLastError());
     launch_post_rebuild(ctl, sb.maxcell, F.write_xrb, stream);
     GCL_HIP(hipGetLastError());
+    launch_nmask<D, R>(B, G, C.dist2, stream);
+    GCL_HIP(hipGetLastError());
+    masks_valid = true;
   }
 
   template <int D, typename R>
   void stepsolve_t(int64_t nsteps, bool ts, double tl0, double tl1, int64_t* updaten, double* maxcdx) {
     SphBufs<R>& B = bufs<R>();
+    ensure_nmask(B);
     const SphConst C = make_const(tl0, tl1);
     const bool dpc_on = par[SPH_DPC_L0] > 0;
     const bool xsph_on = par[SPH_XSPH_EPS] > 0;
@@ -495,7 +532,7 @@ struct sph_s {
       xrb_mode = need_xrb ? 0 : 1;
       p_from_dpc = dpc_on;
     }
-    launch_diag<D, R>(B, C, G, xrb_mode, stream);
+    launch_diag<D, R>(B, C, G, xrb_mode, masks_valid ? 1 : 0, stream);
     GCL_HIP(hipGetLastError());
     read_ctl();
     if (timing) collect_timing();
@@ -540,6 +577,7 @@ struct sph_s {
   void user_rebuild_t() {
     SphBufs<R>& B = bufs<R>();
     read_ctl();
+    ensure_nmask(B);
     hctl.rebuild = 1;
     write_ctl();
     StepFlags F{};

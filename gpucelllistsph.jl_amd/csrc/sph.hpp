@@ -43,6 +43,11 @@ struct StepFlags {
   int write_xrb;    // the rebuild stores its positions for later membership tests
 };
 
+// Position arrays (both parities and the rebuild copy) carry this many zero
+// elements past n: the membership test reads candidates in groups of four and
+// may run past the end of the last stencil row.
+constexpr int64_t kPosPad = 64;
+
 // Device buffers of one problem.  Main state is double-buffered ([parity]);
 // ptype / id are double-buffered by the reorder only ([pparity]).
 template <typename R>
@@ -77,6 +82,12 @@ struct SphBufs {
   StepCtl* ctl;
   double* RED;  // per-block partial maxima: [|cdx|, |a|, visc] x nblk
   int nblk;
+  // neighbour-list membership of the last rebuild (k_nmask): entry e of
+  // particle a at NE[e * n + a] = (window base << 32) | member bits; NC[a]
+  // entries, or -1 when more than necap were needed
+  uint64_t* NE;
+  int32_t* NC;
+  int necap;
 };
 
 template <int D, typename R>
@@ -96,7 +107,10 @@ void launch_cspm(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, cons
 template <int D, typename R>
 void launch_dt_visc(const SphBufs<R>& B, const SphConst& C, hipStream_t s);
 template <int D, typename R>
-void launch_diag(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int xm_mode, hipStream_t s);
+void launch_diag(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int xm_mode, int use_masks,
+                 hipStream_t s);
+template <int D, typename R>
+void launch_nmask(const SphBufs<R>& B, const CellGeom& G, double dist2, hipStream_t s);
 void launch_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb, hipStream_t s);
 void launch_finalize(StepCtl* ctl, const double* red, int nblk, const SphConst& C, int ts, int corr_on, hipStream_t s);
 

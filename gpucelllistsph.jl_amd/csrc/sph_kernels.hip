@@ -180,55 +180,153 @@ __device__ __forceinline__ void membership_ptrs(const SphBufs<R>& B, const StepF
   for (int k = 0; k < 3; ++k) XM[k] = cur ? B.X[p][k < D ? k : 1] : B.XRB[k < D ? k : 1];
 }
 
-// Visits the neighbour-list partners b of particle a: candidates of the 3^D
-// stencil rows passing the membership test.  Candidates are taken in windows of
-// 64 per row; the members of a window are first recorded as bit masks and then
-// visited, so all lanes of a wave run the (long) pair body together instead of
-// idling on the roughly two thirds of candidates that are not partners.
-template <int D, typename F>
-__device__ __forceinline__ void for_each_neighbor(const Rows<D>& rows, int64_t a, const double* const* XM,
-                                                  double dist2, F&& body) {
-  constexpr int NR = Rows<D>::N;
-  const double xa0 = XM[0][a], xa1 = XM[1][a], xa2 = D == 3 ? XM[2][a] : 0.0;
-  for (int32_t w0 = 0;; w0 += 64) {
-    unsigned long long m[NR];
-    int32_t base[NR];
-    bool more = false;
-#pragma unroll
-    for (int k = 0; k < NR; ++k) {
-      const int32_t lo = rows.b[k] + w0;
-      const int32_t hi = rows.e[k] < lo + 64 ? rows.e[k] : lo + 64;
-      unsigned long long mk = 0;
-      for (int32_t b = lo; b < hi; ++b) {
+// Particle index of this thread with an XCD-aware block order.  Workgroups are
+// dealt to the 8 XCDs round robin (block b runs on XCD b % 8); renumbering so
+// that each XCD walks one contiguous range of cell-sorted particles keeps the
+// neighbourhoods shared by consecutive blocks (and by the stencil rows above
+// and below) inside one XCD's L2 instead of fetching them on every XCD.
+__device__ __forceinline__ int64_t xcd_particle_index() {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned q = G / 8, r = G % 8, xcd = b % 8, i = b / 8;
+  const unsigned L = xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
+  return int64_t(L) * blockDim.x + threadIdx.x;
+}
+
+// Member bits of one window [lo, hi) (hi - lo <= 32) of a stencil row: bit j is
+// set when candidate lo + j is a partner of particle a, i.e. b != a and
+// r^2 < dist^2 with the reference's expression (no FMA).  Candidates are read
+// four at a time so their loads are in flight together; reads run up to 3 past
+// hi, which the position arrays' padding keeps in bounds, and those bits are
+// dropped.
+template <int D>
+__device__ __forceinline__ uint32_t window_bits(const double* const* XM, double xa0, double xa1, double xa2,
+                                                int64_t a, int32_t lo, int32_t hi, double dist2) {
 #pragma clang fp contract(off)
-        const double d0 = xa0 - XM[0][b];
-        const double d1 = xa1 - XM[1][b];
-        double r2 = d0 * d0 + d1 * d1;
-        if (D == 3) {
-          const double d2 = xa2 - XM[2][b];
-          r2 = r2 + d2 * d2;
+  const double* p0 = XM[0] + lo;
+  const double* p1 = XM[1] + lo;
+  const double* p2 = XM[2] + lo;
+  const int32_t len = hi - lo;
+  uint32_t mk = 0;
+  for (int32_t j = 0; j < len; j += 4) {
+    double c0[4], c1[4], c2[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c0[u] = p0[j + u];
+      c1[u] = p1[j + u];
+      if (D == 3) c2[u] = p2[j + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double d0 = xa0 - c0[u];
+      const double d1 = xa1 - c1[u];
+      double r2 = d0 * d0 + d1 * d1;
+      if (D == 3) {
+        const double d2 = xa2 - c2[u];
+        r2 = r2 + d2 * d2;
+      }
+      mk |= uint32_t(r2 < dist2) << (j + u);
+    }
+  }
+  if (len < 32) mk &= (1u << len) - 1u;
+  const int64_t self = a - lo;
+  if (self >= 0 && self < len) mk &= ~(1u << self);
+  return mk;
+}
+
+// Sorted range [b, e) of stencil row k (dy = k%3 - 1, dz = k/3 - 1) of cell c;
+// empty when the row lies outside the grid.
+template <int D>
+__device__ __forceinline__ void row_range(const CellGeom& g, const int32_t* __restrict__ start, int32_t c, int k,
+                                          int32_t& b, int32_t& e) {
+  const int64_t gx = g.grid[0], gy = g.grid[1];
+  const int64_t ix = c % gx, rest = c / gx;
+  const int64_t iy = rest % gy, iz = rest / gy;
+  const int64_t xl = ix > 0 ? ix - 1 : 0, xh = ix + 1 < gx ? ix + 1 : gx - 1;
+  const int64_t y = iy + (k % 3) - 1;
+  const int64_t z = D == 3 ? iz + (k / 3) - 1 : 0;
+  const bool in = y >= 0 && y < gy && (D == 2 || (z >= 0 && z < g.grid[2]));
+  const int64_t row = gx * (y + gy * z);
+  b = in ? start[row + xl] : 0;
+  e = in ? start[row + xh + 1] : 0;
+}
+
+// Neighbour-list membership of the last rebuild, stored per particle as
+// (mask, base) entries: one 32-candidate window [base, base+32) of one stencil
+// row with its member bits in the low word.  Only windows with members are
+// stored, in row order.  Entry e of particle a is NE[e * n + a]; NC[a] is the
+// entry count, or -1 when the particle needs more than necap entries (its
+// partners are then found by scanning the stencil).
+template <int D, typename R>
+__global__ __launch_bounds__(256) void k_nmask(SphBufs<R> B, CellGeom G, double dist2) {
+  const StepCtl* ctl = B.ctl;
+  if (!ctl->rebuilt) return;
+  const int64_t a = xcd_particle_index();
+  if (a >= B.n) return;
+  const int p = ctl->parity;  // the rebuild positions are the current ones
+  const double* XM[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
+  const double xa0 = XM[0][a], xa1 = XM[1][a], xa2 = D == 3 ? XM[2][a] : 0.0;
+  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  const int cap = B.necap;
+  uint64_t* out = B.NE + a;
+  int cnt = 0;
+#pragma unroll
+  for (int k = 0; k < Rows<D>::N; ++k) {
+    for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
+      const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
+      const uint32_t mk = window_bits<D>(XM, xa0, xa1, xa2, a, w0, hi, dist2);
+      if (mk != 0) {
+        if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
+        ++cnt;
+      }
+    }
+  }
+  B.NC[a] = cnt <= cap ? cnt : -1;
+}
+
+// Visits the neighbour-list partners b of particle a in a fixed order (stencil
+// rows in order, ascending sorted index inside a row).  nc >= 0: the nc stored
+// entries of k_nmask, read one ahead so the next load overlaps the current
+// window's pair bodies.  nc < 0: the windows are computed here from the
+// membership positions XM.  Each lane runs its own sequence, so all lanes of a
+// wave execute the (long) pair body together.
+template <int D, typename R, typename F>
+__device__ __forceinline__ void for_each_partner(const SphBufs<R>& B, int64_t a, int nc, const CellGeom& G,
+                                                 const double* const* XM, double dist2, F&& body) {
+  const uint64_t* pe = B.NE + a;
+  uint64_t nxt = nc > 0 ? *pe : 0ull;
+  int e = 1;
+  const bool scan = nc < 0;
+  int k = -1;
+  int32_t w0 = 0, re = 0;
+  uint32_t c = 0;
+  int32_t cb = 0;
+  for (;;) {
+    if (c == 0) {
+      if (!scan) {
+        if (nxt == 0) break;  // stored entries always have member bits
+        c = uint32_t(nxt);
+        cb = int32_t(nxt >> 32);
+        pe += B.n;
+        nxt = e < nc ? *pe : 0ull;
+        ++e;
+      } else {
+        while (c == 0) {
+          if (w0 >= re) {
+            if (++k >= Rows<D>::N) break;
+            row_range<D>(G, B.start, B.skey[a], k, w0, re);
+            continue;
+          }
+          const int32_t hi = re < w0 + 32 ? re : w0 + 32;
+          c = window_bits<D>(XM, XM[0][a], XM[1][a], D == 3 ? XM[2][a] : 0.0, a, w0, hi, dist2);
+          cb = w0;
+          w0 = hi;
         }
-        if (r2 < dist2 && b != a) mk |= 1ull << (b - lo);
+        if (c == 0) break;
       }
-      m[k] = mk;
-      base[k] = lo;
-      more |= rows.e[k] > lo + 64;
     }
-    int k = 0;
-    unsigned long long c = m[0];
-    int32_t cb = base[0];
-    for (;;) {
-      while (c == 0 && k < NR - 1) {
-        ++k;
-        c = select_k(m, k);
-        cb = select_k(base, k);
-      }
-      if (c == 0) break;
-      const int bit = __builtin_ctzll(c);
-      c &= c - 1;
-      body(cb + bit);
-    }
-    if (!more) break;
+    const int bit = __builtin_ctz(c);
+    c &= c - 1;
+    body(cb + bit);
   }
 }
 
@@ -254,18 +352,6 @@ __device__ __forceinline__ void block_max(double v, bool active, double* slots) 
   __syncthreads();
 }
 
-
-// Particle index of this thread with an XCD-aware block order.  Workgroups are
-// dealt to the 8 XCDs round robin (block b runs on XCD b % 8); renumbering so
-// that each XCD walks one contiguous range of cell-sorted particles keeps the
-// neighbourhoods shared by consecutive blocks (and by the stencil rows above
-// and below) inside one XCD's L2 instead of fetching them on every XCD.
-__device__ __forceinline__ int64_t xcd_particle_index() {
-  const unsigned G = gridDim.x, b = blockIdx.x;
-  const unsigned q = G / 8, r = G % 8, xcd = b % 8, i = b / 8;
-  const unsigned L = xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
-  return int64_t(L) * blockDim.x + threadIdx.x;
-}
 
 // ------------------------------------------------------------------ kernels
 // Runs after the (conditional) rebuild kernels: commits the reorder.
@@ -342,11 +428,11 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
   const int32_t pta = PT[a];
   Sum<R> acc[3], drho;
 
-  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  const int nc = B.NC[a];
   if constexpr (std::is_same<R, double>::value) {
     const double rhoa = sa;
     const double Pa = B.PN[a];
-    for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+    for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
       const int32_t ptb = PT[b];
       double dx[3], dv[3], gw[3];
 #pragma unroll
@@ -447,7 +533,7 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
     const float rhoa = C.f_rho0 + dra;
     const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
     const float Pa = B.PN[a];
-    for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+    for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
       const int32_t ptb = PT[b];
       float dx[3], dv[3], gw[3];
 #pragma unroll
@@ -579,13 +665,13 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
     const R sha = RH[a];
     const int32_t pta = PT[a];
     Sum<R> acc[3], drho;
-    const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
+    const int nc = B.NC[a];
     R av[3];
     R Pha, drho_v;
     if constexpr (std::is_same<R, double>::value) {
       const double rhoa = sha;
       Pha = B.PH[a];
-      for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+      for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
         const int32_t ptb = PT[b];
         double dxn[3], dx[3], dv[3], gw[3];
 #pragma unroll
@@ -695,7 +781,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
       const float rhoa = C.f_rho0 + dra;
       const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
       Pha = B.PH[a];
-      for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+      for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
         const int32_t ptb = PT[b];
         float dxn[3], dx[3], dv[3], gw[3];
 #pragma unroll
@@ -841,8 +927,8 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SphBufs<R> B, SphConst C, CellG
   const double rhoa = std::is_same<R, double>::value ? double(RHO[a]) : C.rho0 + double(RHO[a]);
   const double Pa = double(B.PN[a]);
   double sum[3] = {0.0, 0.0, 0.0};
-  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
-  for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+  const int nc = B.NC[a];
+  for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
       const int32_t ptb = PT[b];
       if (!(pta > 0 && ptb > 0)) return;
       double dx[3], dv[3];
@@ -915,8 +1001,8 @@ __global__ __launch_bounds__(256) void k_xsph(SphBufs<R> B, SphConst C, CellGeom
   membership_ptrs<D>(B, F, p, rebuilt, XM);
   const double rhoa = std::is_same<R, double>::value ? double(RHO[a]) : C.rho0 + double(RHO[a]);
   double sum[3] = {0.0, 0.0, 0.0};
-  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
-  for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+  const int nc = B.NC[a];
+  for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
       if (PT[b] <= 1) return;
       double dx[3];
 #pragma unroll
@@ -948,8 +1034,8 @@ __global__ __launch_bounds__(256) void k_cspm_sum(SphBufs<R> B, SphConst C, Cell
   membership_ptrs<D>(B, F, p, rebuilt, XM);
   double s1 = 0.0, s2 = 0.0;
   if (pta > 0) {
-    const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
-    for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+    const int nc = B.NC[a];
+    for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
         if (PT[b] <= 0) return;
         double dx[3];
 #pragma unroll
@@ -1053,7 +1139,7 @@ __global__ void k_finalize(StepCtl* ctl, double* __restrict__ red, SphConst C, i
 
 // sph∑W! + sph∑∇W! over the current positions (src/gpukernels_sph.jl:139-156, :332-359).
 template <int D, typename R>
-__global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom G, int xm_mode) {
+__global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom G, int xm_mode, int use_masks) {
 #pragma clang fp contract(off)
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, pp = ctl->pparity;
@@ -1069,8 +1155,8 @@ __global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom
   const int32_t* PT = B.PT[pp];
   const bool pa = PT[a] > 0;
   double sw = 0.0, sg[3] = {0.0, 0.0, 0.0};
-  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
-  for_each_neighbor<D>(rows, a, XM, C.dist2, [&](const int32_t b) {
+  const int nc = use_masks ? B.NC[a] : -1;
+  for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
       if (pa != (PT[b] > 0)) return;
       double dx[3];
 #pragma unroll
@@ -1124,8 +1210,13 @@ void launch_dt_visc(const SphBufs<R>& B, const SphConst& C, hipStream_t s) {
   k_dt_visc<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C);
 }
 template <int D, typename R>
-void launch_diag(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int xm_mode, hipStream_t s) {
-  k_diag<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, xm_mode);
+void launch_diag(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int xm_mode, int use_masks,
+                 hipStream_t s) {
+  k_diag<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, xm_mode, use_masks);
+}
+template <int D, typename R>
+void launch_nmask(const SphBufs<R>& B, const CellGeom& G, double dist2, hipStream_t s) {
+  k_nmask<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, G, dist2);
 }
 void launch_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb, hipStream_t s) {
   k_post_rebuild<<<1, 64, 0, s>>>(ctl, maxcell, write_xrb);
@@ -1148,7 +1239,8 @@ void launch_finalize(StepCtl* ctl, const double* red, int nblk, const SphConst& 
   template void launch_cspm<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,       \
                                   hipStream_t);                                                                \
   template void launch_dt_visc<D, R>(const SphBufs<R>&, const SphConst&, hipStream_t);                         \
-  template void launch_diag<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, int, hipStream_t);
+  template void launch_diag<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, int, int, hipStream_t);  \
+  template void launch_nmask<D, R>(const SphBufs<R>&, const CellGeom&, double, hipStream_t);
 
 GCLSPH_INSTANTIATE(2, double)
 GCLSPH_INSTANTIATE(3, double)

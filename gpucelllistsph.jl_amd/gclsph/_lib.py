@@ -10,7 +10,8 @@ import os
 import subprocess
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_ROOT, "lib", "libgclsph.so")
+# GCLSPH_LIB names another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("GCLSPH_LIB") or os.path.join(PKG_ROOT, "lib", "libgclsph.so")
 
 GCLSPH_F64 = 64
 GCLSPH_F32 = 32

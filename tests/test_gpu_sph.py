@@ -140,6 +140,35 @@ def test_deterministic(G, S):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32split"])
+@pytest.mark.parametrize("case", ["dam2d_allphys", "dam3d_allphys"])
+def test_stored_membership_equals_stencil_scan(G, monkeypatch, prec, case):
+    """Partners read from the per-particle membership store and partners found
+    by scanning the stencil (the path of particles whose windows exceed the
+    store's capacity, forced here with GCLSPH_NMASK_CAP) visit the same pairs in
+    the same order: the steps are bit-identical, including steps that skip the
+    rebuild (dist = 1.1 H) and every correction."""
+    sc, steps, ts = golden_cases()[case]
+
+    def run(cap):
+        if cap is None:
+            monkeypatch.delenv("GCLSPH_NMASK_CAP", raising=False)
+        else:
+            monkeypatch.setenv("GCLSPH_NMASK_CAP", str(cap))
+        cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, v=sc.v, precision=prec)
+        upd, mx = G.stepsolve_(prob, steps + 3, timestepping=ts)
+        G.update_(cl)  # a user rebuild between calls refreshes the store too
+        G.stepsolve_(prob, 2, timestepping=ts)
+        return [upd, mx] + [np.asarray(gpu_fields(G, prob)[f]) for f in FIELDS]
+
+    ref = run(None)
+    for cap in (0, 2):
+        got = run(cap)
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b)
+
+
 def test_user_update_and_set_points_between_calls(G, O, S):
     """update!(prob.system), partialupdate! and edits of prob.system.points between stepsolve! calls."""
     sc = golden_cases()["dam2d_allphys"][0]
