@@ -8,8 +8,8 @@ layers (47,991 particles), hydrostatic initial density, dist = H (a rebuild
 every step), adaptive CFL = 0.2 Δt.  A step is one `stepsolve!` iteration over
 the whole scene with all state resident in HBM.
 
-The headline `value` is the FP32-split path (FP32 pair sums with Kahan
-compensation, density as ρ-ρ0, FP64 positions and Δt); the FP64 path runs on
+The headline `value` is the FP32-split path (FP32 pair terms summed in FP64,
+density as ρ-ρ0, FP64 positions and Δt); the FP64 path runs on
 the same GPU in the same invocation and is reported beside it.
 
 Prints one JSON line.

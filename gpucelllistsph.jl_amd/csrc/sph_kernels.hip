@@ -19,8 +19,8 @@
 // R = double: the Float64 path.  Every expression is evaluated in the
 // reference's operation order with FMA contraction off, so branch decisions
 // (r < r0, cond < 0, clamps) see the same values as the reference would.
-// R = float: the FP32-split path (density stored as ρ-ρ0, Kahan pair sums,
-// Float64 positions and Δt).
+// R = float: the FP32-split path (density stored as ρ-ρ0, FP32 pair terms
+// summed in FP64, Float64 positions and Δt).
 #include <type_traits>
 
 #include "sph.hpp"
@@ -131,15 +131,10 @@ struct Sum {  // plain sum
   __device__ __forceinline__ R get() const { return s; }
 };
 template <>
-struct Sum<float> {  // Kahan-compensated sum
-  float s = 0.0f, c = 0.0f;
-  __device__ __forceinline__ void add(float v) {
-    float y = v - c;
-    float t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ __forceinline__ float get() const { return s; }
+struct Sum<float> {  // FP32 pair terms accumulated in FP64 (2 VALU per term, exact for ~30 terms)
+  double s = 0.0;
+  __device__ __forceinline__ void add(float v) { s += double(v); }
+  __device__ __forceinline__ float get() const { return float(s); }
 };
 
 template <int D>
@@ -283,27 +278,34 @@ __global__ __launch_bounds__(256) void k_nmask(SphBufs<R> B, CellGeom G, double 
   B.NC[a] = cnt <= cap ? cnt : -1;
 }
 
-// Visits the neighbour-list partners b of particle a in a fixed order (stencil
-// rows in order, ascending sorted index inside a row).  nc >= 0: the nc stored
-// entries of k_nmask, read one ahead so the next load overlaps the current
-// window's pair bodies.  nc < 0: the windows are computed here from the
-// membership positions XM.  Each lane runs its own sequence, so all lanes of a
-// wave execute the (long) pair body together.
-template <int D, typename R, typename F>
-__device__ __forceinline__ void for_each_partner(const SphBufs<R>& B, int64_t a, int nc, const CellGeom& G,
-                                                 const double* const* XM, double dist2, F&& body) {
-  const uint64_t* pe = B.NE + a;
-  uint64_t nxt = nc > 0 ? *pe : 0ull;
-  int e = 1;
-  const bool scan = nc < 0;
-  int k = -1;
-  int32_t w0 = 0, re = 0;
+// Neighbour-list partners b of particle a in a fixed order (stencil rows in
+// order, ascending sorted index inside a row).  nc >= 0: the nc stored entries
+// of k_nmask, read one ahead so the next load overlaps the current window's
+// pair bodies.  nc < 0: the windows are computed here from the membership
+// positions XM.  Each lane runs its own sequence, so all lanes of a wave
+// execute the (long) pair body together.
+template <int D, typename R>
+struct Partners {
+  const SphBufs<R>& B;
+  const CellGeom& G;
+  const double* const* XM;
+  const double dist2;
+  const int64_t a;
+  const int nc;
+  const uint64_t* pe;
+  uint64_t nxt;
+  int e = 1, k = -1;
+  int32_t w0 = 0, re = 0, cb = 0;
   uint32_t c = 0;
-  int32_t cb = 0;
-  for (;;) {
+
+  __device__ __forceinline__ Partners(const SphBufs<R>& B_, int64_t a_, int nc_, const CellGeom& G_,
+                                      const double* const* XM_, double dist2_)
+      : B(B_), G(G_), XM(XM_), dist2(dist2_), a(a_), nc(nc_), pe(B_.NE + a_), nxt(nc_ > 0 ? *pe : 0ull) {}
+
+  __device__ __forceinline__ bool next(int32_t& b) {
     if (c == 0) {
-      if (!scan) {
-        if (nxt == 0) break;  // stored entries always have member bits
+      if (nc >= 0) {
+        if (nxt == 0) return false;  // stored entries always have member bits
         c = uint32_t(nxt);
         cb = int32_t(nxt >> 32);
         pe += B.n;
@@ -321,12 +323,47 @@ __device__ __forceinline__ void for_each_partner(const SphBufs<R>& B, int64_t a,
           cb = w0;
           w0 = hi;
         }
-        if (c == 0) break;
+        if (c == 0) return false;
       }
     }
-    const int bit = __builtin_ctz(c);
+    b = cb + __builtin_ctz(c);
     c &= c - 1;
-    body(cb + bit);
+    return true;
+  }
+};
+
+template <int D, typename R, typename F>
+__device__ __forceinline__ void for_each_partner(const SphBufs<R>& B, int64_t a, int nc, const CellGeom& G,
+                                                 const double* const* XM, double dist2, F&& body) {
+  Partners<D, R> it(B, a, nc, G, XM, dist2);
+  int32_t b;
+  while (it.next(b)) body(b);
+}
+
+// As for_each_partner, with the partner data split off: fetch(b) loads what
+// the body needs of partner b.  Pipe: the fetch of the next partner is issued
+// before the body of the current one runs, so the loads overlap the pair
+// arithmetic (at the cost of the registers holding both partners' data).
+template <bool Pipe, int D, typename R, typename Fetch, typename F>
+__device__ __forceinline__ void for_each_partner_fetch(const SphBufs<R>& B, int64_t a, int nc, const CellGeom& G,
+                                                       const double* const* XM, double dist2, Fetch&& fetch,
+                                                       F&& body) {
+  Partners<D, R> it(B, a, nc, G, XM, dist2);
+  int32_t b;
+  if constexpr (!Pipe) {
+    while (it.next(b)) body(b, fetch(b));
+    return;
+  }
+  if (!it.next(b)) return;
+  auto cur = fetch(b);
+  for (;;) {
+    int32_t nb = b;
+    const bool more = it.next(nb);
+    const auto nxt = fetch(nb);  // the last partner again when there is none: a valid address
+    body(b, cur);
+    if (!more) break;
+    b = nb;
+    cur = nxt;
   }
 }
 
@@ -432,13 +469,29 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
   if constexpr (std::is_same<R, double>::value) {
     const double rhoa = sa;
     const double Pa = B.PN[a];
-    for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
-      const int32_t ptb = PT[b];
+    struct Nb {
+      double x[3], v[3], rho, P;
+      int32_t pt;
+    };
+    auto fetch = [&](const int32_t b) {
+      Nb q;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        q.x[d] = X[d][b];
+        q.v[d] = V[d][b];
+      }
+      q.rho = RHO[b];
+      q.P = B.PN[b];
+      q.pt = PT[b];
+      return q;
+    };
+    for_each_partner_fetch<false, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+      const int32_t ptb = q.pt;
       double dx[3], dv[3], gw[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        dx[d] = xa[d] - X[d][b];
-        dv[d] = va[d] - V[d][b];
+        dx[d] = xa[d] - q.x[d];
+        dv[d] = va[d] - q.v[d];
       }
       const double r2 = dotd<D>(dx, dx);
       const double r = sqrt(r2);
@@ -446,7 +499,7 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
       const double dwk_r = dW_d(u, C.dwnorm) / r;
 #pragma unroll
       for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
-      const double rhob = RHO[b];
+      const double rhob = q.rho;
       const bool both = pta > 0 && ptb > 0;
       double ax[3] = {0.0, 0.0, 0.0};
       if (both) {  // ∂ρ∂tDDT! (:579-625)
@@ -460,7 +513,7 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
         drho.add(s);
       }
       if ((long long)pta * (long long)ptb > 0) {  // ∂v∂t! (:755-772)
-        double Pfac = (Pa + B.PN[b]) / (rhoa * rhob);
+        double Pfac = (Pa + q.P) / (rhoa * rhob);
 #pragma unroll
         for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
       }
@@ -533,13 +586,30 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
     const float rhoa = C.f_rho0 + dra;
     const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
     const float Pa = B.PN[a];
-    for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
-      const int32_t ptb = PT[b];
+    struct Nb {
+      double x[3];
+      float v[3], rho, P;
+      int32_t pt;
+    };
+    auto fetch = [&](const int32_t b) {
+      Nb q;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        q.x[d] = X[d][b];
+        q.v[d] = V[d][b];
+      }
+      q.rho = RHO[b];
+      q.P = B.PN[b];
+      q.pt = PT[b];
+      return q;
+    };
+    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+      const int32_t ptb = q.pt;
       float dx[3], dv[3], gw[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        dx[d] = float(xa[d] - X[d][b]);
-        dv[d] = va[d] - V[d][b];
+        dx[d] = float(xa[d] - q.x[d]);
+        dv[d] = va[d] - q.v[d];
       }
       const float r2 = dotf<D>(dx, dx);
       const float rinv = __builtin_amdgcn_rsqf(r2);
@@ -549,7 +619,7 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
       const float inv_den = __builtin_amdgcn_rcpf(r2 + C.f_eta2);
 #pragma unroll
       for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
-      const float drb = RHO[b];
+      const float drb = q.rho;
       const float rhob = C.f_rho0 + drb;
       const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
       const bool both = pta > 0 && ptb > 0;
@@ -565,7 +635,7 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
         drho.add(s);
       }
       if ((long long)pta * (long long)ptb > 0) {
-        float Pfac = (Pa + B.PN[b]) * (inv_rhoa * inv_rhob);
+        float Pfac = (Pa + q.P) * (inv_rhoa * inv_rhob);
 #pragma unroll
         for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
       }
@@ -671,14 +741,31 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
     if constexpr (std::is_same<R, double>::value) {
       const double rhoa = sha;
       Pha = B.PH[a];
-      for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
-        const int32_t ptb = PT[b];
+      struct Nb {
+        double xn[3], xh[3], vh[3], rh, P;
+        int32_t pt;
+      };
+      auto fetch = [&](const int32_t b) {
+        Nb q;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          q.xn[d] = X[d][b];
+          q.xh[d] = XH[d][b];
+          q.vh[d] = VH[d][b];
+        }
+        q.rh = RH[b];
+        q.P = B.PH[b];
+        q.pt = PT[b];
+        return q;
+      };
+      for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+        const int32_t ptb = q.pt;
         double dxn[3], dx[3], dv[3], gw[3];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          dxn[d] = xa[d] - X[d][b];
-          dx[d] = xha[d] - XH[d][b];
-          dv[d] = vha[d] - VH[d][b];
+          dxn[d] = xa[d] - q.xn[d];
+          dx[d] = xha[d] - q.xh[d];
+          dv[d] = vha[d] - q.vh[d];
         }
         // ∇W from pass 1 (positions xⁿ, src/sphproblem.jl:246,281)
         const double r2n = dotd<D>(dxn, dxn);
@@ -687,7 +774,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
 #pragma unroll
         for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
         const double r2 = dotd<D>(dx, dx);
-        const double rhob = RH[b];
+        const double rhob = q.rh;
         const bool both = pta > 0 && ptb > 0;
         double ax[3] = {0.0, 0.0, 0.0};
         if (both) {
@@ -701,7 +788,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
           drho.add(s);
         }
         if ((long long)pta * (long long)ptb > 0) {
-          double Pfac = (Pha + B.PH[b]) / (rhoa * rhob);
+          double Pfac = (Pha + q.P) / (rhoa * rhob);
 #pragma unroll
           for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
         }
@@ -781,14 +868,32 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
       const float rhoa = C.f_rho0 + dra;
       const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
       Pha = B.PH[a];
-      for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
-        const int32_t ptb = PT[b];
+      struct Nb {
+        double xn[3], xh[3];
+        float vh[3], rh, P;
+        int32_t pt;
+      };
+      auto fetch = [&](const int32_t b) {
+        Nb q;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          q.xn[d] = X[d][b];
+          q.xh[d] = XH[d][b];
+          q.vh[d] = VH[d][b];
+        }
+        q.rh = RH[b];
+        q.P = B.PH[b];
+        q.pt = PT[b];
+        return q;
+      };
+      for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+        const int32_t ptb = q.pt;
         float dxn[3], dx[3], dv[3], gw[3];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          dxn[d] = float(xa[d] - X[d][b]);
-          dx[d] = float(xha[d] - XH[d][b]);
-          dv[d] = vha[d] - VH[d][b];
+          dxn[d] = float(xa[d] - q.xn[d]);
+          dx[d] = float(xha[d] - q.xh[d]);
+          dv[d] = vha[d] - q.vh[d];
         }
         const float r2n = dotf<D>(dxn, dxn);
         const float rninv = __builtin_amdgcn_rsqf(r2n);
@@ -798,7 +903,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
         for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
         const float r2 = dotf<D>(dx, dx);
         const float inv_den = __builtin_amdgcn_rcpf(r2 + C.f_eta2);
-        const float drb = RH[b];
+        const float drb = q.rh;
         const float rhob = C.f_rho0 + drb;
         const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
         const bool both = pta > 0 && ptb > 0;
@@ -814,7 +919,7 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
           drho.add(s);
         }
         if ((long long)pta * (long long)ptb > 0) {
-          float Pfac = (Pha + B.PH[b]) * (inv_rhoa * inv_rhob);
+          float Pfac = (Pha + q.P) * (inv_rhoa * inv_rhob);
 #pragma unroll
           for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
         }

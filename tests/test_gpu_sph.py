@@ -4,7 +4,7 @@ Tolerances.  FP64 path: per-pair terms are evaluated in the reference's
 operation order without FMA contraction, so the only difference from the
 oracle is the order of the per-particle sums (the reference's own FP64 atomic
 order is nondeterministic); fields agree to ~1e-13 relative after tens of
-steps.  FP32-split path: pair sums in FP32 (Kahan), density as ρ-ρ0, FP64
+steps.  FP32-split path: FP32 pair terms summed in FP64, density as ρ-ρ0, FP64
 positions; bounds follow the acceptance target of 1e-5 relative density and
 1e-4·dx positions after 1000 steps."""
 import math
@@ -258,11 +258,15 @@ def test_full_size_f32split_vs_f64_1000_steps(G, S):
     xp, _, _ = run("f64", 4e-16)
     x32, r32, t32 = run("f32split")
     dx = sc.params.dx
-    floor = np.abs(xp - x64).max()
+    chaos = np.abs(xp - x64).max(axis=0)
+    floor = chaos.max()
     err = np.abs(x32 - x64).max(axis=0)
     print(f"density max rel {np.abs(r32 / r64 - 1).max():.3e}; position/dx max {err.max() / dx:.3e} "
-          f"p99.99 {np.quantile(err, 0.9999) / dx:.3e}; FP64 chaos floor/dx {floor / dx:.3e}")
+          f"p99.99 {np.quantile(err, 0.9999) / dx:.3e}; FP64 chaos floor/dx {floor / dx:.3e} "
+          f"p99.99 {np.quantile(chaos, 0.9999) / dx:.3e}")
     assert np.abs(r32 / r64 - 1).max() < 1e-5
     assert np.quantile(err, 0.9999) < 1e-4 * dx
+    # FP32-split deviates less than FP64 does from its own 1-ulp perturbation
+    assert np.quantile(err, 0.9999) <= np.quantile(chaos, 0.9999)
     assert err.max() <= floor
     assert t32 == pytest.approx(t64, rel=1e-6)
