@@ -63,22 +63,22 @@ def run_gpu(scene, precision: str, steps: int, warmup: int):
     return {"seconds": t1 - t0, "updaten": upd, "timing": timing, "dt": prob.dt}
 
 
-def read_traffic(path: str):
-    """HBM bytes per dispatch of the dominant kernel, from the committed
-    rocprofv3 PMC summary (FETCH_SIZE / WRITE_SIZE scaled by the copy-kernel
-    calibration), or None."""
-    if not os.path.exists(path):
-        return None
+def read_profile(path: str) -> dict:
+    """`key: value` lines of the committed rocprofv3 PMC summary of the dominant
+    kernel: HBM bytes per dispatch (FETCH_SIZE / WRITE_SIZE scaled by the
+    copy-kernel calibration) and the VALU busy fraction."""
     vals = {}
+    if not os.path.exists(path):
+        return vals
     with open(path) as fh:
         for line in fh:
-            if ":" in line and line.strip().startswith("traffic_bytes_per_dispatch"):
-                k, v = line.split(":", 1)
+            k, sep, v = line.partition(":")
+            if sep and k.strip() in ("traffic_bytes_per_dispatch", "valu_busy_per_dispatch"):
                 try:
                     vals[k.strip()] = float(v.split()[0])
-                except ValueError:
+                except (ValueError, IndexError):
                     pass
-    return vals.get("traffic_bytes_per_dispatch")
+    return vals
 
 
 def cpu_baseline(scene, seconds_budget: float = 60.0):
@@ -137,7 +137,8 @@ def main():
     dom = max(("pass1", "pass2"), key=lambda k: stages[k][0])
     dom_ms = stages[dom][0] / max(stages[dom][1], 1)
     achieved = bytes_pp[dom] * n / (dom_ms * 1e-3) / 1e9
-    traffic = read_traffic(args.traffic_file) if dom == "pass2" else None
+    prof = read_profile(args.traffic_file) if dom == "pass2" else {}
+    traffic = prof.get("traffic_bytes_per_dispatch")
     out = {
         "metric": "particle-steps/s (full SPH step) FP32-split vs FP64 on 1 MI355X; HBM GB/s",
         "value": value,
@@ -168,7 +169,9 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
             "bytes_per_particle": bytes_pp[dom],
-            "note": "pair passes are VALU-bound; algorithmic bytes / measured kernel time",
+            "valu_busy": prof.get("valu_busy_per_dispatch"),
+            "note": "pair passes are VALU-bound: achieved = algorithmic bytes / measured kernel time; "
+                    "valu_busy (share of SIMD cycles issuing VALU) and traffic come from the committed PMC profile",
         },
     }
     if res64 is not None:
