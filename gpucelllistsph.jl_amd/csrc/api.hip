@@ -202,6 +202,10 @@ struct sph_s {
   uint64_t* ne_buf = nullptr;  // neighbour-mask entries, necap per particle
   int necap = 0;
   bool masks_valid = false;  // NE/NC hold the membership of the last rebuild
+  // sumW / sum∇W of the last stepsolve call are computed on first use (or
+  // before anything changes the state they describe)
+  bool diag_pending = false;
+  int diag_xm = 0, diag_masks = 0;
   bool timing = false;
   double stage_ms[5] = {0, 0, 0, 0, 0};
   int64_t stage_cnt[5] = {0, 0, 0, 0, 0};
@@ -475,6 +479,7 @@ struct sph_s {
   template <int D, typename R>
   void stepsolve_t(int64_t nsteps, bool ts, double tl0, double tl1, int64_t* updaten, double* maxcdx) {
     SphBufs<R>& B = bufs<R>();
+    diag_pending = false;  // replaced by this call's
     ensure_nmask(B);
     const SphConst C = make_const(tl0, tl1);
     const bool dpc_on = par[SPH_DPC_L0] > 0;
@@ -532,8 +537,9 @@ struct sph_s {
       xrb_mode = need_xrb ? 0 : 1;
       p_from_dpc = dpc_on;
     }
-    launch_diag<D, R>(B, C, G, xrb_mode, masks_valid ? 1 : 0, stream);
-    GCL_HIP(hipGetLastError());
+    diag_pending = true;
+    diag_xm = xrb_mode;
+    diag_masks = masks_valid ? 1 : 0;
     read_ctl();
     if (timing) collect_timing();
     par[SPH_DT] = hctl.dt;
@@ -616,7 +622,25 @@ struct sph_s {
     sys->mppcell = hctl.mppcell;
     sys->pairs_valid = false;
   }
+  template <int D, typename R>
+  void flush_diag_t() {
+    launch_diag<D, R>(bufs<R>(), make_const(0.0, 0.0), G, diag_xm, diag_masks, stream);
+    GCL_HIP(hipGetLastError());
+  }
+  void flush_diag() {
+    if (!diag_pending) return;
+    diag_pending = false;
+    if (split()) {
+      if (dim == 2) flush_diag_t<2, float>();
+      else flush_diag_t<3, float>();
+    } else {
+      if (dim == 2) flush_diag_t<2, double>();
+      else flush_diag_t<3, double>();
+    }
+  }
+
   void user_rebuild() {
+    flush_diag();
     if (split()) {
       if (dim == 2) user_rebuild_t<2, float>();
       else user_rebuild_t<3, float>();
@@ -851,6 +875,7 @@ int gcl_set_points(gcl_t h, const double* const* pts) {
     if (h->attached) {
       sph_s* P = h->attached;
       P->read_ctl();
+      P->flush_diag();
       P->materialize_halfstep();
       double* tmp = nullptr;
       GCL_HIP(hipMalloc(&tmp, 8 * size_t(h->n)));
@@ -1005,6 +1030,9 @@ int sph_create(gcl_t system, int prec, const double* params, const double* rho, 
     GCL_CHECK(g >= 1 && g <= 64 && std::floor(g) == g, kErrArgument, "gamma must be an integer in [1, 64]");
     GCL_CHECK(prec != GCLSPH_F32SPLIT || g == 7, kErrUnsupported, "the FP32-split path implements gamma = 7 only");
     GCL_CHECK(params[SPH_H] > 0 && params[SPH_HH] > 0, kErrArgument, "h and H must be positive");
+    // pair kernels address per-particle arrays with 32-bit byte offsets
+    GCL_CHECK(system->n < (int64_t(1) << 29) - kPosPad, kErrUnsupported,
+              "an SPH problem holds fewer than 2^29 - 64 particles");
     require_device();
     std::unique_ptr<sph_s> P(new sph_s());
     P->sys = system;
@@ -1097,6 +1125,7 @@ int sph_set_param(sph_t h, int which, double value) {
               "gamma must be an integer");
     GCL_CHECK(which != SPH_GAMMA || !h->split() || value == 7, kErrUnsupported,
               "the FP32-split path implements gamma = 7 only");
+    h->flush_diag();
     h->par[which] = value;
   });
 }
@@ -1151,9 +1180,11 @@ int sph_get_field(sph_t h, int field, double* out) {
         else vec_r(h->Bd.ACC, 0.0);
         break;
       case SPH_FIELD_SUMW:
+        h->flush_diag();
         put(unsort<double>(sp ? h->Bf.SUMW : h->Bd.SUMW, id, n, h->stream), 0);
         break;
       case SPH_FIELD_SUMGRADW:
+        h->flush_diag();
         vec_r(sp ? h->Bf.SUMGW : h->Bd.SUMGW, 0.0);
         break;
       case SPH_FIELD_DRHO:

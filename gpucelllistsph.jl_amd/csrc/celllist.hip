@@ -83,18 +83,25 @@ __global__ void k_cellmap(XPtrs xp, XPtrs xalt, const int* sel, int64_t n, CellG
   }
   // Input already in (old) cell order: lanes of a wave mostly share a few
   // cells, so one atomic per distinct cell of the wave serves all its lanes.
+  // The peer groups are found first (register-only loop); the group leaders
+  // then issue their atomics together, so the wave waits for one round trip.
   const int lane = threadIdx.x & 63;
-  unsigned long long todo = __ballot(1);
+  unsigned long long todo = __ballot(1), mine = 0;
+  int leader = lane;
   while (todo) {
-    const int leader = __ffsll((long long)todo) - 1;
-    const int32_t lk = __shfl(k, leader, 64);
+    const int l = __ffsll((long long)todo) - 1;
+    const int32_t lk = __shfl(k, l, 64);
     const unsigned long long peers = __ballot(k == lk) & todo;
-    int32_t base = 0;
-    if (lane == leader) base = atomicAdd(&cnt[lk], __popcll(peers));
-    base = __shfl(base, leader, 64);
-    if (k == lk) slot[i] = base + __popcll(peers & ((1ull << lane) - 1));
+    if (k == lk) {
+      mine = peers;
+      leader = l;
+    }
     todo &= ~peers;
   }
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(&cnt[lin], __popcll(mine));
+  base = __shfl(base, leader, 64);
+  slot[i] = base + __popcll(mine & ((1ull << lane) - 1));
 }
 
 __global__ void k_scatter(const int32_t* __restrict__ key, const int32_t* __restrict__ slot,

@@ -278,6 +278,15 @@ __global__ __launch_bounds__(256) void k_nmask(SphBufs<R> B, CellGeom G, double 
   B.NC[a] = cnt <= cap ? cnt : -1;
 }
 
+// Element of a per-particle array at a 32-bit byte offset: the load then uses
+// the scalar base + 32-bit vector offset addressing form (no 64-bit address
+// arithmetic per partner).  Arrays stay below 4 GiB (the problem size limit
+// checked at construction).
+template <typename T>
+__device__ __forceinline__ T ldo(const T* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
 // Neighbour-list partners b of particle a in a fixed order (stencil rows in
 // order, ascending sorted index inside a row).  nc >= 0: the nc stored entries
 // of k_nmask, read one ahead so the next load overlaps the current window's
@@ -349,21 +358,26 @@ __device__ __forceinline__ void for_each_partner_fetch(const SphBufs<R>& B, int6
                                                        const double* const* XM, double dist2, Fetch&& fetch,
                                                        F&& body) {
   Partners<D, R> it(B, a, nc, G, XM, dist2);
-  int32_t b;
+  int32_t b0, b1;
   if constexpr (!Pipe) {
-    while (it.next(b)) body(b, fetch(b));
+    while (it.next(b0)) body(b0, fetch(b0));
     return;
   }
-  if (!it.next(b)) return;
-  auto cur = fetch(b);
+  // two partners in flight, alternating between two sets of registers; a
+  // missing next partner fetches the current one again (a valid address)
+  if (!it.next(b0)) return;
+  auto f0 = fetch(b0);
   for (;;) {
-    int32_t nb = b;
-    const bool more = it.next(nb);
-    const auto nxt = fetch(nb);  // the last partner again when there is none: a valid address
-    body(b, cur);
-    if (!more) break;
-    b = nb;
-    cur = nxt;
+    b1 = b0;
+    const bool m1 = it.next(b1);
+    auto f1 = fetch(b1);
+    body(b0, f0);
+    if (!m1) break;
+    b0 = b1;
+    const bool m0 = it.next(b0);
+    f0 = fetch(b0);
+    body(b1, f1);
+    if (!m0) break;
   }
 }
 
@@ -474,15 +488,16 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
       int32_t pt;
     };
     auto fetch = [&](const int32_t b) {
+      const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
       Nb q;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        q.x[d] = X[d][b];
-        q.v[d] = V[d][b];
+        q.x[d] = ldo(X[d], o8);
+        q.v[d] = ldo(V[d], oR);
       }
-      q.rho = RHO[b];
-      q.P = B.PN[b];
-      q.pt = PT[b];
+      q.rho = ldo(RHO, oR);
+      q.P = ldo<R>(B.PN, oR);
+      q.pt = ldo(PT, o4);
       return q;
     };
     for_each_partner_fetch<false, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
@@ -592,15 +607,16 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
       int32_t pt;
     };
     auto fetch = [&](const int32_t b) {
+      const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
       Nb q;
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        q.x[d] = X[d][b];
-        q.v[d] = V[d][b];
+        q.x[d] = ldo(X[d], o8);
+        q.v[d] = ldo(V[d], oR);
       }
-      q.rho = RHO[b];
-      q.P = B.PN[b];
-      q.pt = PT[b];
+      q.rho = ldo(RHO, oR);
+      q.P = ldo<R>(B.PN, oR);
+      q.pt = ldo(PT, o4);
       return q;
     };
     for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
@@ -746,16 +762,17 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
         int32_t pt;
       };
       auto fetch = [&](const int32_t b) {
+        const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
         Nb q;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          q.xn[d] = X[d][b];
-          q.xh[d] = XH[d][b];
-          q.vh[d] = VH[d][b];
+          q.xn[d] = ldo(X[d], o8);
+          q.xh[d] = ldo(XH[d], o8);
+          q.vh[d] = ldo(VH[d], oR);
         }
-        q.rh = RH[b];
-        q.P = B.PH[b];
-        q.pt = PT[b];
+        q.rh = ldo(RH, oR);
+        q.P = ldo<R>(B.PH, oR);
+        q.pt = ldo(PT, o4);
         return q;
       };
       for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
@@ -874,16 +891,17 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
         int32_t pt;
       };
       auto fetch = [&](const int32_t b) {
+        const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
         Nb q;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          q.xn[d] = X[d][b];
-          q.xh[d] = XH[d][b];
-          q.vh[d] = VH[d][b];
+          q.xn[d] = ldo(X[d], o8);
+          q.xh[d] = ldo(XH[d], o8);
+          q.vh[d] = ldo(VH[d], oR);
         }
-        q.rh = RH[b];
-        q.P = B.PH[b];
-        q.pt = PT[b];
+        q.rh = ldo(RH, oR);
+        q.P = ldo<R>(B.PH, oR);
+        q.pt = ldo(PT, o4);
         return q;
       };
       for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
