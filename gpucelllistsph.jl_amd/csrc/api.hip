@@ -206,6 +206,9 @@ struct sph_s {
   // before anything changes the state they describe)
   bool diag_pending = false;
   int diag_xm = 0, diag_masks = 0;
+  LaunchSet own;     // this problem as a batch of one
+  std::vector<unsigned char> own_desc;  // its descriptor (passed by value to the kernels)
+  int batches = 0;   // batches that step this problem
   bool timing = false;
   double stage_ms[5] = {0, 0, 0, 0, 0};
   int64_t stage_cnt[5] = {0, 0, 0, 0, 0};
@@ -219,6 +222,7 @@ struct sph_s {
     for (void* p : allocs)
       if (p) (void)hipFree(p);
     if (ne_buf) (void)hipFree(ne_buf);
+    own.release();
     sb.release();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
@@ -429,16 +433,16 @@ struct sph_s {
     }
     return ev_pool[ev_used++];
   }
-  hipEvent_t mark_begin() {
+  hipEvent_t mark_begin(hipStream_t st) {
     if (!timing) return nullptr;
     hipEvent_t e = take_event();
-    GCL_HIP(hipEventRecord(e, stream));
+    GCL_HIP(hipEventRecord(e, st));
     return e;
   }
-  void mark_end(int stage, hipEvent_t b) {
+  void mark_end(int stage, hipEvent_t b, hipStream_t st) {
     if (!timing) return;
     hipEvent_t e = take_event();
-    GCL_HIP(hipEventRecord(e, stream));
+    GCL_HIP(hipEventRecord(e, st));
     ev_marks.push_back({stage, {b, e}});
   }
   void collect_timing() {
@@ -452,7 +456,8 @@ struct sph_s {
     ev_used = 0;
   }
 
-  void write_ctl() { GCL_HIP(hipMemcpyAsync(ctl, &hctl, sizeof(StepCtl), hipMemcpyHostToDevice, stream)); }
+  void write_ctl() { write_ctl(stream); }
+  void write_ctl(hipStream_t st) { GCL_HIP(hipMemcpyAsync(ctl, &hctl, sizeof(StepCtl), hipMemcpyHostToDevice, st)); }
   void read_ctl() {
     GCL_HIP(hipMemcpyAsync(&hctl, ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, stream));
     GCL_HIP(hipStreamSynchronize(stream));
@@ -461,104 +466,54 @@ struct sph_s {
   template <typename R>
   SphBufs<R>& bufs();
 
-  template <int D, typename R>
-  void rebuild_stage(const SphConst& C, const StepFlags& F) {
-    SphBufs<R>& B = bufs<R>();
-    const void* x0[3] = {B.X[0][0], B.X[0][1], B.X[0][2]};
-    const void* x1[3] = {B.X[1][0], B.X[1][1], B.X[1][2]};
-    bin_sort(G, 64, x0, n, false, sb, &ctl->rebuild, stream, x1, &ctl->parity);
-    launch_reorder<D, R>(B, C, F, stream);
-    GCL_HIP(hipGetLastError());
-    launch_post_rebuild(ctl, sb.maxcell, F.write_xrb, stream);
-    GCL_HIP(hipGetLastError());
-    launch_nmask<D, R>(B, G, C.dist2, stream);
-    GCL_HIP(hipGetLastError());
-    masks_valid = true;
-  }
-
-  template <int D, typename R>
-  void stepsolve_t(int64_t nsteps, bool ts, double tl0, double tl1, int64_t* updaten, double* maxcdx) {
-    SphBufs<R>& B = bufs<R>();
-    diag_pending = false;  // replaced by this call's
-    ensure_nmask(B);
-    const SphConst C = make_const(tl0, tl1);
-    const bool dpc_on = par[SPH_DPC_L0] > 0;
-    const bool xsph_on = par[SPH_XSPH_EPS] > 0;
-    const int64_t cspmn = int64_t(par[SPH_CSPMN]);
-    const bool need_xrb = par[SPH_NUI] > 0 || dpc_on || xsph_on;
-    hctl.rebuild = 1;  // the first iteration of every call rebuilds (src/sphproblem.jl:212,222)
-    hctl.updaten = 0;
-    hctl.maxcdx_out = 0;
-    hctl.dt = par[SPH_DT];
-    hctl.etime = par[SPH_ETIME];
-    hctl.red_cdx = hctl.red_acc = hctl.red_visc = 0;
-    write_ctl();
+  // Per-call switches of this problem (the per-step ones follow on the device).
+  StepFlags call_flags(bool ts) const {
+    const bool dpc_on = par[SPH_DPC_L0] > 0, xsph_on = par[SPH_XSPH_EPS] > 0;
     StepFlags F{};
-    F.need_xrb = need_xrb ? 1 : 0;
+    F.need_xrb = (par[SPH_NUI] > 0 || dpc_on || xsph_on) ? 1 : 0;
     F.need_cdx = par[SPH_NUI] > 0 ? 1 : 0;
     F.ts = ts ? 1 : 0;
     F.visc_in_pass2 = (ts && !dpc_on && !xsph_on) ? 1 : 0;
     F.nu_on = par[SPH_NU] > 0 ? 1 : 0;
     F.s_on = par[SPH_S] > 0 ? 1 : 0;
-    F.write_xrb = need_xrb ? 1 : 0;
-    for (int64_t it = 1; it <= nsteps; ++it) {
-      const bool cspm_fire = cspmn > 0 && it % cspmn == 0;
-      const bool corr_on = dpc_on || xsph_on || cspm_fire;
-      F.synced_prev = synced_prev ? 1 : 0;
-      F.corr_on = corr_on ? 1 : 0;
-      F.want_out = it == nsteps ? 1 : 0;
-      hipEvent_t e0 = mark_begin();
-      rebuild_stage<D, R>(C, F);
-      mark_end(0, e0);
-      e0 = mark_begin();
-      launch_pass1<D, R>(B, C, G, F, stream);
-      GCL_HIP(hipGetLastError());
-      mark_end(1, e0);
-      e0 = mark_begin();
-      launch_pass2<D, R>(B, C, G, F, stream);
-      GCL_HIP(hipGetLastError());
-      mark_end(2, e0);
-      if (corr_on || (ts && !F.visc_in_pass2)) {
-        e0 = mark_begin();
-        if (dpc_on) launch_dpc<D, R>(B, C, G, F, stream);
-        if (xsph_on) launch_xsph<D, R>(B, C, G, F, stream);
-        if (cspm_fire) launch_cspm<D, R>(B, C, G, F, stream);
-        if (ts && !F.visc_in_pass2) launch_dt_visc<D, R>(B, C, stream);
-        GCL_HIP(hipGetLastError());
-        mark_end(3, e0);
-      }
-      e0 = mark_begin();
-      launch_finalize(ctl, B.RED, B.nblk, C, F.ts, F.corr_on, stream);
-      GCL_HIP(hipGetLastError());
-      mark_end(4, e0);
-      synced_prev = !corr_on;
-    }
-    if (nsteps > 0) {
-      xrb_mode = need_xrb ? 0 : 1;
-      p_from_dpc = dpc_on;
-    }
-    diag_pending = true;
-    diag_xm = xrb_mode;
-    diag_masks = masks_valid ? 1 : 0;
-    read_ctl();
-    if (timing) collect_timing();
-    par[SPH_DT] = hctl.dt;
-    par[SPH_ETIME] = hctl.etime;
-    *updaten = hctl.updaten;
-    *maxcdx = hctl.maxcdx_out;
-    sys->mppcell = hctl.mppcell;
-    sys->pairs_valid = false;
+    F.write_xrb = F.need_xrb;
+    F.dpc_on = dpc_on ? 1 : 0;
+    F.xsph_on = xsph_on ? 1 : 0;
+    const int64_t cspmn = int64_t(par[SPH_CSPMN]);  // src/sphproblem.jl:319
+    F.cspmn = cspmn > 0 ? int(std::min<int64_t>(cspmn, std::numeric_limits<int>::max())) : 0;
+    return F;
   }
 
-  void stepsolve(int64_t nsteps, bool ts, double tl0, double tl1, int64_t* updaten, double* maxcdx) {
-    if (split()) {
-      if (dim == 2) stepsolve_t<2, float>(nsteps, ts, tl0, tl1, updaten, maxcdx);
-      else stepsolve_t<3, float>(nsteps, ts, tl0, tl1, updaten, maxcdx);
-    } else {
-      if (dim == 2) stepsolve_t<2, double>(nsteps, ts, tl0, tl1, updaten, maxcdx);
-      else stepsolve_t<3, double>(nsteps, ts, tl0, tl1, updaten, maxcdx);
-    }
+  template <typename R>
+  SceneDev<R> scene_desc(const SphConst& C, const StepFlags& F) {
+    SceneDev<R> d{};
+    d.B = bufs<R>();
+    d.C = C;
+    d.G = G;
+    d.F = F;
+    d.key = sb.key;
+    d.slot = sb.slot;
+    d.cnt = sb.cnt;
+    d.start = sb.start;
+    d.tmp = sb.tmp;
+    d.perm = sb.perm;
+    d.skey = sb.skey;
+    d.maxcell = sb.maxcell;
+    d.scan_tmp = sb.scan_tmp;
+    d.ncells = G.ncells();
+    return d;
   }
+
+  // Descriptor of this problem for its own one-problem launch set.
+  template <typename R>
+  void upload_own(const SphConst& C, const StepFlags& F) {
+    const SceneDev<R> d = scene_desc<R>(C, F);
+    own_desc.resize(sizeof(d));
+    std::memcpy(own_desc.data(), &d, sizeof(d));
+    own.one = own_desc.data();
+  }
+
+  void stepsolve(int64_t nsteps, bool ts, double tl0, double tl1, int64_t* updaten, double* maxcdx);
 
   // Membership positions of the current pair list (host pointer choice).
   const double* membership(int k) {
@@ -585,11 +540,14 @@ struct sph_s {
     read_ctl();
     ensure_nmask(B);
     hctl.rebuild = 1;
+    hctl.synced = synced_prev ? 1 : 0;
     write_ctl();
-    StepFlags F{};
-    F.synced_prev = synced_prev ? 1 : 0;
+    StepFlags F = call_flags(false);
     F.write_xrb = 1;
-    rebuild_stage<D, R>(make_const(0.0, 0.0), F);
+    upload_own<R>(make_const(0.0, 0.0), F);
+    launch_rebuild<D, R>(own, stream);
+    GCL_HIP(hipGetLastError());
+    masks_valid = true;
     // per-particle arrays outside the reorder follow the new order
     auto permute_d = [&](double* a) {
       double* t = B.TMP;
@@ -624,7 +582,8 @@ struct sph_s {
   }
   template <int D, typename R>
   void flush_diag_t() {
-    launch_diag<D, R>(bufs<R>(), make_const(0.0, 0.0), G, diag_xm, diag_masks, stream);
+    upload_own<R>(make_const(0.0, 0.0), call_flags(false));
+    launch_diag<D, R>(own, diag_xm, diag_masks, stream);
     GCL_HIP(hipGetLastError());
   }
   void flush_diag() {
@@ -678,6 +637,169 @@ template <>
 SphBufs<float>& sph_s::bufs<float>() {
   return Bf;
 }
+
+void LaunchSet::build(const int64_t* n, const int64_t* ncells, unsigned cnt, size_t bytes) {
+  release();
+  count = cnt;
+  scene_bytes = bytes;
+  std::vector<int2> pm, cm;
+  for (unsigned i = 0; i < cnt; ++i) {
+    const int64_t nb = (n[i] + kBlock - 1) / kBlock;
+    for (int64_t b = 0; b < nb; ++b) pm.push_back(make_int2(int(i), int(b)));
+    const int64_t nt = (ncells[i] + 1 + 1023) / 1024;  // cnt[0..ncells] in 1024-cell tiles
+    for (int64_t t = 0; t < nt; ++t) cm.push_back(make_int2(int(i), int(t)));
+  }
+  if (pm.empty()) pm.push_back(make_int2(0, 0));  // a no-op block (a >= n everywhere)
+  pblocks = unsigned(pm.size());
+  ctiles = unsigned(cm.size());
+  GCL_HIP(hipMalloc(&scenes, bytes * cnt));
+  GCL_HIP(hipMalloc(&pmap, sizeof(int2) * pm.size()));
+  GCL_HIP(hipMalloc(&cmap, sizeof(int2) * cm.size()));
+  GCL_HIP(hipMemcpy(pmap, pm.data(), sizeof(int2) * pm.size(), hipMemcpyHostToDevice));
+  GCL_HIP(hipMemcpy(cmap, cm.data(), sizeof(int2) * cm.size(), hipMemcpyHostToDevice));
+}
+
+void LaunchSet::release() {
+  if (scenes) (void)hipFree(scenes);
+  if (pmap) (void)hipFree(pmap);
+  if (cmap) (void)hipFree(cmap);
+  scenes = nullptr;
+  pmap = cmap = nullptr;
+  pblocks = ctiles = count = 0;
+}
+
+namespace {
+
+// `stepsolve!` (src/sphproblem.jl:207-345) for count problems stepped
+// together: every stage is one launch over all of them, each problem keeping
+// its own control block (Δt, rebuilds, time, corrections).  The problems'
+// results are those of stepping each on its own.
+template <int D, typename R>
+void run_steps(sph_s* const* P, unsigned count, LaunchSet& L, hipStream_t st, int64_t nsteps, bool ts,
+               const double* tl0, const double* tl1, int64_t* updaten, double* maxcdx) {
+  GCL_CHECK(nsteps <= std::numeric_limits<int>::max(), kErrArgument, "too many steps in one call");
+  std::vector<SceneDev<R>> descs(count);
+  std::vector<StepFlags> flags(count);
+  bool any_dpc = false, any_xsph = false, any_dtv = false;
+  for (unsigned i = 0; i < count; ++i) {
+    sph_s* p = P[i];
+    SphBufs<R>& B = p->bufs<R>();
+    p->diag_pending = false;  // replaced by this call's
+    p->ensure_nmask(B);
+    const StepFlags F = p->call_flags(ts);
+    flags[i] = F;
+    descs[i] = p->scene_desc<R>(p->make_const(tl0[i], tl1[i]), F);
+    any_dpc |= F.dpc_on != 0;
+    any_xsph |= F.xsph_on != 0;
+    any_dtv |= F.ts && !F.visc_in_pass2;
+    StepCtl& h = p->hctl;
+    h.rebuild = 1;  // the first iteration of every call rebuilds (src/sphproblem.jl:212,222)
+    h.updaten = 0;
+    h.maxcdx_out = 0;
+    h.dt = p->par[SPH_DT];
+    h.etime = p->par[SPH_ETIME];
+    h.red_cdx = h.red_acc = h.red_visc = 0;
+    h.iter = 1;
+    h.nsteps = int(nsteps);
+    h.synced = p->synced_prev ? 1 : 0;
+    p->write_ctl(st);
+  }
+  // one problem: the descriptor goes by value with every launch
+  if (count == 1) L.one = descs.data();
+  else GCL_HIP(hipMemcpyAsync(L.scenes, descs.data(), sizeof(SceneDev<R>) * count, hipMemcpyHostToDevice, st));
+  sph_s* timer = (count == 1 && P[0]->timing) ? P[0] : nullptr;
+  auto mark_begin = [&]() { return timer ? timer->mark_begin(st) : hipEvent_t(nullptr); };
+  auto mark_end = [&](int stage, hipEvent_t e) {
+    if (timer) timer->mark_end(stage, e, st);
+  };
+  for (int64_t it = 1; it <= nsteps; ++it) {
+    bool any_cspm = false;
+    for (unsigned i = 0; i < count; ++i) any_cspm |= flags[i].cspmn > 0 && it % flags[i].cspmn == 0;
+    hipEvent_t e0 = mark_begin();
+    launch_rebuild<D, R>(L, st);
+    GCL_HIP(hipGetLastError());
+    mark_end(0, e0);
+    e0 = mark_begin();
+    launch_pass1<D, R>(L, st);
+    GCL_HIP(hipGetLastError());
+    mark_end(1, e0);
+    e0 = mark_begin();
+    launch_pass2<D, R>(L, st);
+    GCL_HIP(hipGetLastError());
+    mark_end(2, e0);
+    if (any_dpc || any_xsph || any_cspm || any_dtv) {
+      e0 = mark_begin();
+      if (any_dpc) launch_dpc<D, R>(L, st);
+      if (any_xsph) launch_xsph<D, R>(L, st);
+      if (any_cspm) launch_cspm<D, R>(L, st);
+      if (any_dtv) launch_dt_visc<D, R>(L, st);
+      GCL_HIP(hipGetLastError());
+      mark_end(3, e0);
+    }
+    e0 = mark_begin();
+    launch_finalize<D, R>(L, st);
+    GCL_HIP(hipGetLastError());
+    mark_end(4, e0);
+    for (unsigned i = 0; i < count; ++i) {
+      const StepFlags& F = flags[i];
+      const bool cspm_fire = F.cspmn > 0 && it % F.cspmn == 0;
+      P[i]->synced_prev = !(F.dpc_on || F.xsph_on || cspm_fire);
+    }
+  }
+  for (unsigned i = 0; i < count; ++i) {
+    sph_s* p = P[i];
+    if (nsteps > 0) {
+      p->xrb_mode = flags[i].need_xrb ? 0 : 1;
+      p->p_from_dpc = flags[i].dpc_on != 0;
+      p->masks_valid = true;
+    }
+    p->diag_pending = true;
+    p->diag_xm = p->xrb_mode;
+    p->diag_masks = p->masks_valid ? 1 : 0;
+    GCL_HIP(hipMemcpyAsync(&p->hctl, p->ctl, sizeof(StepCtl), hipMemcpyDeviceToHost, st));
+  }
+  GCL_HIP(hipStreamSynchronize(st));
+  if (count == 1) L.one = nullptr;
+  if (timer) timer->collect_timing();
+  for (unsigned i = 0; i < count; ++i) {
+    sph_s* p = P[i];
+    p->par[SPH_DT] = p->hctl.dt;
+    p->par[SPH_ETIME] = p->hctl.etime;
+    updaten[i] = p->hctl.updaten;
+    maxcdx[i] = p->hctl.maxcdx_out;
+    p->sys->mppcell = p->hctl.mppcell;
+    p->sys->pairs_valid = false;
+  }
+}
+
+template <int D, typename R>
+void run_steps_one(sph_s* p, int64_t nsteps, bool ts, double tl0, double tl1, int64_t* updaten, double* maxcdx) {
+  run_steps<D, R>(&p, 1, p->own, p->stream, nsteps, ts, &tl0, &tl1, updaten, maxcdx);
+}
+
+}  // namespace
+
+void sph_s::stepsolve(int64_t nsteps, bool ts, double tl0, double tl1, int64_t* updaten, double* maxcdx) {
+  if (split()) {
+    if (dim == 2) run_steps_one<2, float>(this, nsteps, ts, tl0, tl1, updaten, maxcdx);
+    else run_steps_one<3, float>(this, nsteps, ts, tl0, tl1, updaten, maxcdx);
+  } else {
+    if (dim == 2) run_steps_one<2, double>(this, nsteps, ts, tl0, tl1, updaten, maxcdx);
+    else run_steps_one<3, double>(this, nsteps, ts, tl0, tl1, updaten, maxcdx);
+  }
+}
+
+// A set of problems stepped together (one launch per stage for all of them).
+struct sph_batch_s {
+  std::vector<sph_s*> probs;
+  LaunchSet L;
+  hipStream_t stream = nullptr;
+  ~sph_batch_s() {
+    for (sph_s* p : probs) p->batches -= 1;
+    L.release();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
 
 namespace {
 
@@ -1090,8 +1212,22 @@ int sph_create(gcl_t system, int prec, const double* params, const double* rho, 
     P->hctl.dt = P->par[SPH_DT];
     P->hctl.etime = P->par[SPH_ETIME];
     P->write_ctl();
-    if (prec == GCLSPH_F32SPLIT) launch_init_pn<float>(P->Bf, P->make_const(0.0, 0.0), P->stream);
-    else launch_init_pn<double>(P->Bd, P->make_const(0.0, 0.0), P->stream);
+    {
+      const int64_t nc = P->G.ncells();
+      const SphConst C0 = P->make_const(0.0, 0.0);
+      const StepFlags F0 = P->call_flags(false);
+      if (prec == GCLSPH_F32SPLIT) {
+        P->own.build(&n, &nc, 1, sizeof(SceneDev<float>));
+        P->upload_own<float>(C0, F0);
+        if (P->dim == 2) launch_init_pn<2, float>(P->own, P->stream);
+        else launch_init_pn<3, float>(P->own, P->stream);
+      } else {
+        P->own.build(&n, &nc, 1, sizeof(SceneDev<double>));
+        P->upload_own<double>(C0, F0);
+        if (P->dim == 2) launch_init_pn<2, double>(P->own, P->stream);
+        else launch_init_pn<3, double>(P->own, P->stream);
+      }
+    }
     GCL_HIP(hipGetLastError());
     GCL_HIP(hipStreamSynchronize(P->stream));
     P->xrb_mode = 0;
@@ -1104,6 +1240,7 @@ int sph_create(gcl_t system, int prec, const double* params, const double* rho, 
 int sph_destroy(sph_t h) {
   return guard([&] {
     GCL_CHECK(h != nullptr, kErrArgument, "NULL handle");
+    GCL_CHECK(h->batches == 0, kErrState, "problem belongs to a batch (destroy the batch first)");
     // hand the current points back to the cell list (caller order)
     gcl_s* g = h->sys;
     h->read_ctl();
@@ -1219,6 +1356,66 @@ int sph_get_timing(sph_t h, double* ms, int64_t* launches, int cap, int* nstages
       if (launches) launches[k] = h->stage_cnt[k];
     }
     if (nstages) *nstages = 5;
+  });
+}
+
+int sph_batch_create(const sph_t* problems, int32_t count, sph_batch_t* out) {
+  return guard([&] {
+    GCL_CHECK(problems != nullptr && out != nullptr, kErrArgument, "NULL argument");
+    GCL_CHECK(count > 0, kErrArgument, "a batch needs at least one problem");
+    *out = nullptr;
+    for (int32_t i = 0; i < count; ++i) {
+      GCL_CHECK(problems[i] != nullptr, kErrArgument, "NULL problem in batch");
+      GCL_CHECK(problems[i]->dim == problems[0]->dim && problems[i]->prec == problems[0]->prec, kErrArgument,
+                "batched problems must share dimension and precision");
+      for (int32_t j = 0; j < i; ++j)
+        GCL_CHECK(problems[j] != problems[i], kErrArgument, "a problem appears twice in the batch");
+    }
+    require_device();
+    std::unique_ptr<sph_batch_s> b(new sph_batch_s());
+    b->probs.assign(problems, problems + count);
+    for (sph_s* p : b->probs) p->batches += 1;
+    GCL_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    std::vector<int64_t> n(static_cast<size_t>(count)), nc(static_cast<size_t>(count));
+    for (int32_t i = 0; i < count; ++i) {
+      n[size_t(i)] = b->probs[size_t(i)]->n;
+      nc[size_t(i)] = b->probs[size_t(i)]->G.ncells();
+    }
+    const size_t bytes = problems[0]->split() ? sizeof(SceneDev<float>) : sizeof(SceneDev<double>);
+    b->L.build(n.data(), nc.data(), unsigned(count), bytes);
+    *out = b.release();
+  });
+}
+
+int sph_batch_destroy(sph_batch_t b) {
+  return guard([&] {
+    GCL_CHECK(b != nullptr, kErrArgument, "NULL handle");
+    delete b;
+  });
+}
+
+int sph_batch_stepsolve(sph_batch_t b, int64_t n, int timestepping, const double* tl0, const double* tl1,
+                        int64_t* updaten, double* maxcdx) {
+  return guard([&] {
+    GCL_CHECK(b != nullptr && updaten != nullptr && maxcdx != nullptr, kErrArgument, "NULL argument");
+    GCL_CHECK(n >= 0, kErrArgument, "number of steps must be >= 0");
+    const size_t cnt = b->probs.size();
+    std::vector<double> t0(cnt), t1(cnt);
+    for (size_t i = 0; i < cnt; ++i) {
+      const double* par = b->probs[i]->par;
+      // default timelims (sqrt(eps), CFL·H/(3c0)) (src/sphproblem.jl:207,426)
+      t0[i] = tl0 ? tl0[i] : std::sqrt(std::numeric_limits<double>::epsilon());
+      t1[i] = tl1 ? tl1[i] : par[SPH_CFL] * par[SPH_HH] / (3 * par[SPH_C0]);
+    }
+    sph_s* const* P = b->probs.data();
+    const bool ts = timestepping != 0;
+    if (P[0]->split()) {
+      if (P[0]->dim == 2) run_steps<2, float>(P, unsigned(cnt), b->L, b->stream, n, ts, t0.data(), t1.data(), updaten, maxcdx);
+      else run_steps<3, float>(P, unsigned(cnt), b->L, b->stream, n, ts, t0.data(), t1.data(), updaten, maxcdx);
+    } else {
+      if (P[0]->dim == 2) run_steps<2, double>(P, unsigned(cnt), b->L, b->stream, n, ts, t0.data(), t1.data(), updaten, maxcdx);
+      else run_steps<3, double>(P, unsigned(cnt), b->L, b->stream, n, ts, t0.data(), t1.data(), updaten, maxcdx);
+    }
   });
 }
 

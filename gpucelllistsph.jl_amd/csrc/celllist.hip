@@ -13,19 +13,6 @@
 namespace gclsph {
 namespace {
 
-template <typename T>
-struct PrecTraits;
-template <>
-struct PrecTraits<double> {
-  static __host__ __device__ double off(const CellGeom& g, int k) { return g.off64[k]; }
-  static __host__ __device__ double hinv(const CellGeom& g, int k) { return g.hinv64[k]; }
-};
-template <>
-struct PrecTraits<float> {
-  static __host__ __device__ float off(const CellGeom& g, int k) { return g.off32[k]; }
-  static __host__ __device__ float hinv(const CellGeom& g, int k) { return g.hinv32[k]; }
-};
-
 // r^2 as the reference writes it, (a1-b1)^2 + (a2-b2)^2 [+ (a3-b3)^2], each
 // operation rounded on its own (src/gpukernels_cls.jl:326,390,456,527).
 template <typename T, int D>
@@ -39,24 +26,6 @@ __device__ __forceinline__ T dist2_exact(const T* const* x, int64_t a, int64_t b
     s = s + d2 * d2;
   }
   return s;
-}
-
-template <typename T>
-__device__ __forceinline__ int32_t cell_coord(T xv, const CellGeom& g, int k, int use_off64) {
-#pragma clang fp contract(off)
-  double p;
-  if (use_off64) {
-    p = (double(xv) - g.off64[k]) * double(PrecTraits<T>::hinv(g, k));
-  } else {
-    T pt = (xv - PrecTraits<T>::off(g, k)) * PrecTraits<T>::hinv(g, k);
-    p = double(pt);
-  }
-  // min(max(p, 1), grid); a NaN coordinate lands in cell 1 so every index
-  // stays inside the grid (the reference leaves it undefined)
-  double cl = p >= 1.0 ? p : 1.0;
-  double gk = double(g.grid[k]);
-  cl = cl <= gk ? cl : gk;
-  return int32_t(ceil(cl));
 }
 
 struct XPtrs {
@@ -81,27 +50,7 @@ __global__ void k_cellmap(XPtrs xp, XPtrs xalt, const int* sel, int64_t n, CellG
     slot[i] = atomicAdd(&cnt[lin], 1);
     return;
   }
-  // Input already in (old) cell order: lanes of a wave mostly share a few
-  // cells, so one atomic per distinct cell of the wave serves all its lanes.
-  // The peer groups are found first (register-only loop); the group leaders
-  // then issue their atomics together, so the wave waits for one round trip.
-  const int lane = threadIdx.x & 63;
-  unsigned long long todo = __ballot(1), mine = 0;
-  int leader = lane;
-  while (todo) {
-    const int l = __ffsll((long long)todo) - 1;
-    const int32_t lk = __shfl(k, l, 64);
-    const unsigned long long peers = __ballot(k == lk) & todo;
-    if (k == lk) {
-      mine = peers;
-      leader = l;
-    }
-    todo &= ~peers;
-  }
-  int32_t base = 0;
-  if (lane == leader) base = atomicAdd(&cnt[lin], __popcll(mine));
-  base = __shfl(base, leader, 64);
-  slot[i] = base + __popcll(mine & ((1ull << lane) - 1));
+  slot[i] = wave_aggregated_slot(cnt, k);
 }
 
 __global__ void k_scatter(const int32_t* __restrict__ key, const int32_t* __restrict__ slot,

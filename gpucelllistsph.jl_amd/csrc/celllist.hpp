@@ -18,6 +18,64 @@ struct CellGeom {
   __host__ __device__ int64_t ncells() const { return grid[0] * grid[1] * (dim == 3 ? grid[2] : 1); }
 };
 
+template <typename T>
+struct PrecTraits;
+template <>
+struct PrecTraits<double> {
+  static __host__ __device__ double off(const CellGeom& g, int k) { return g.off64[k]; }
+  static __host__ __device__ double hinv(const CellGeom& g, int k) { return g.hinv64[k]; }
+};
+template <>
+struct PrecTraits<float> {
+  static __host__ __device__ float off(const CellGeom& g, int k) { return g.off32[k]; }
+  static __host__ __device__ float hinv(const CellGeom& g, int k) { return g.hinv32[k]; }
+};
+
+// 1-based cell coordinate along axis k (src/gpukernels_cls.jl:29-47): p = (x - off)·(1/cs) in T
+// (or with the Float64 offset), clamped into [1, grid], rounded up.
+template <typename T>
+__device__ __forceinline__ int32_t cell_coord(T xv, const CellGeom& g, int k, int use_off64) {
+#pragma clang fp contract(off)
+  double p;
+  if (use_off64) {
+    p = (double(xv) - g.off64[k]) * double(PrecTraits<T>::hinv(g, k));
+  } else {
+    T pt = (xv - PrecTraits<T>::off(g, k)) * PrecTraits<T>::hinv(g, k);
+    p = double(pt);
+  }
+  // min(max(p, 1), grid); a NaN coordinate lands in cell 1 so every index
+  // stays inside the grid (the reference leaves it undefined)
+  double cl = p >= 1.0 ? p : 1.0;
+  double gk = double(g.grid[k]);
+  cl = cl <= gk ? cl : gk;
+  return int32_t(ceil(cl));
+}
+
+// Arrival slot of key k in its cell's counter: cnt[k] += 1 for every active
+// lane.  Input in (old) cell order makes the lanes of a wave share a few cells,
+// so one atomic per distinct cell of the wave serves all its lanes: the peer
+// groups are found first (a register-only loop), then the group leaders issue
+// their atomics together and the wave waits for one round trip.
+__device__ __forceinline__ int32_t wave_aggregated_slot(int32_t* cnt, int32_t k) {
+  const int lane = threadIdx.x & 63;
+  unsigned long long todo = __ballot(1), mine = 0;
+  int leader = lane;
+  while (todo) {
+    const int l = __ffsll((long long)todo) - 1;
+    const int32_t lk = __shfl(k, l, 64);
+    const unsigned long long peers = __ballot(k == lk) & todo;
+    if (k == lk) {
+      mine = peers;
+      leader = l;
+    }
+    todo &= ~peers;
+  }
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(&cnt[k], __popcll(mine));
+  base = __shfl(base, leader, 64);
+  return base + __popcll(mine & ((1ull << lane) - 1));
+}
+
 // Scratch for one counting sort of n items into ncells cells.
 struct SortBuffers {
   int64_t n = 0, ncells = 0;

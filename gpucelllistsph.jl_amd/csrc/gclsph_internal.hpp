@@ -53,6 +53,8 @@ struct StepCtl {
   int synced;         // half-step state equals the main state (no correction ran)
   int pparity;       // which ptype / id buffer is current (flipped by reorders only)
   int xrb_cur;       // rebuild positions are the current positions (no step since)
+  int iter;          // 1-based iteration inside the running stepsolve call
+  int nsteps;        // iterations of the running call
   long long updaten;  // rebuilds in this call
   long long mppcell;  // reference-visible max-particles-per-cell bookkeeping
   double dt;

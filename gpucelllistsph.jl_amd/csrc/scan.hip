@@ -3,44 +3,10 @@
 // of the tile sums, then the tile scans with their prefix added.  Every kernel
 // takes an optional device flag and returns at once while it is zero, so a
 // rebuild that the device decides against costs only the launches.
-#include "gclsph_internal.hpp"
+#include "blockscan.hpp"
 
 namespace gclsph {
 namespace {
-
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 4;
-constexpr int kTile = kScanThreads * kScanItems;  // 1024
-
-template <typename T>
-__device__ inline T wave_inclusive_scan(T v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    T t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
-  return v;
-}
-
-// Exclusive block scan over kScanThreads values; returns the thread's exclusive
-// prefix and writes the block total to *total.
-template <typename T>
-__device__ inline T block_exclusive_scan(T v, T* total) {
-  __shared__ T wsum[kScanThreads / 64];
-  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  T inc = wave_inclusive_scan(v, lane);
-  if (lane == 63) wsum[w] = inc;
-  __syncthreads();
-  T pre = 0, tot = 0;
-#pragma unroll
-  for (int k = 0; k < kScanThreads / 64; ++k) {
-    if (k < w) pre += wsum[k];
-    tot += wsum[k];
-  }
-  __syncthreads();
-  *total = tot;
-  return pre + inc - v;
-}
 
 __global__ void k_tile_sums(const int32_t* __restrict__ in, int64_t n, int64_t* __restrict__ tsum,
                             const int* flag, int32_t* max_out) {

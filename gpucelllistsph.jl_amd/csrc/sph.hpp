@@ -30,7 +30,10 @@ struct SphConst {
       f_dwnorm, f_fb_d[2], f_fb_r0[2], f_fb_eta2[2], f_st_2h, f_st_c, f_st_01h, f_xsph_c;
 };
 
-// Per-launch switches decided on the host.
+// Switches of one problem.  The host sets the per-call ones; the per-step ones
+// (synced_prev, want_out, corr_on, cspm_fire) are derived on the device from the
+// control block at every launch (step_flags), so a step's launch sequence is the
+// same for every step and every problem of a batch.
 struct StepFlags {
   int need_xrb;     // membership uses the rebuild positions when a step did not rebuild
   int synced_prev;  // half-step state of the previous step equals the main state
@@ -41,6 +44,9 @@ struct StepFlags {
   int visc_in_pass2;
   int nu_on, s_on;
   int write_xrb;    // the rebuild stores its positions for later membership tests
+  int dpc_on, xsph_on;
+  int cspmn;        // CSPM every cspmn iterations of a call (0: never)
+  int cspm_fire;    // CSPM runs after this step
 };
 
 // Position arrays (both parities and the rebuild copy) carry this many zero
@@ -90,28 +96,58 @@ struct SphBufs {
   int necap;
 };
 
-template <int D, typename R>
-void launch_reorder(const SphBufs<R>& B, const SphConst& C, const StepFlags& F, hipStream_t s);
+// Device descriptor of one problem: everything a kernel needs, read by each
+// block from the scene it works on.
 template <typename R>
-void launch_init_pn(const SphBufs<R>& B, const SphConst& C, hipStream_t s);
+struct SceneDev {
+  SphBufs<R> B;
+  SphConst C;
+  CellGeom G;
+  StepFlags F;
+  // counting-sort buffers of the rebuild
+  int32_t *key, *slot, *cnt, *start, *tmp, *perm, *skey, *maxcell;
+  int64_t* scan_tmp;
+  int64_t ncells;
+};
+
+// Launch geometry of a set of problems stepped together (a batch; a single
+// problem is a batch of one).  Particle kernels run one block per 256 particles
+// of one problem; pmap maps a logical block to (problem, block of the problem).
+// Cell kernels run one block per 1024-cell tile; cmap maps a block to (problem,
+// tile).  Per-problem kernels run one block per problem.
+struct LaunchSet {
+  void* scenes = nullptr;  // device SceneDev<R>[count]
+  const void* one = nullptr;  // host SceneDev<R> passed by value when set (count == 1)
+  int2* pmap = nullptr;
+  int2* cmap = nullptr;
+  unsigned pblocks = 0, ctiles = 0, count = 0;
+  size_t scene_bytes = 0;
+  // n[i] particles and ncells[i] cells of problem i
+  void build(const int64_t* n, const int64_t* ncells, unsigned count, size_t scene_bytes);
+  void release();
+};
+
+// Rebuild stage of one step for every problem whose control block asks for it:
+// binning, scan, scatter, rank, reorder of the state, commit, membership store.
 template <int D, typename R>
-void launch_pass1(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s);
+void launch_rebuild(const LaunchSet& L, hipStream_t s);
 template <int D, typename R>
-void launch_pass2(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s);
+void launch_pass1(const LaunchSet& L, hipStream_t s);
 template <int D, typename R>
-void launch_dpc(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s);
+void launch_pass2(const LaunchSet& L, hipStream_t s);
 template <int D, typename R>
-void launch_xsph(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s);
+void launch_dpc(const LaunchSet& L, hipStream_t s);
 template <int D, typename R>
-void launch_cspm(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s);
+void launch_xsph(const LaunchSet& L, hipStream_t s);
 template <int D, typename R>
-void launch_dt_visc(const SphBufs<R>& B, const SphConst& C, hipStream_t s);
+void launch_cspm(const LaunchSet& L, hipStream_t s);
 template <int D, typename R>
-void launch_diag(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int xm_mode, int use_masks,
-                 hipStream_t s);
+void launch_dt_visc(const LaunchSet& L, hipStream_t s);
 template <int D, typename R>
-void launch_nmask(const SphBufs<R>& B, const CellGeom& G, double dist2, hipStream_t s);
-void launch_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb, hipStream_t s);
-void launch_finalize(StepCtl* ctl, const double* red, int nblk, const SphConst& C, int ts, int corr_on, hipStream_t s);
+void launch_finalize(const LaunchSet& L, hipStream_t s);
+template <int D, typename R>
+void launch_diag(const LaunchSet& L, int xm_mode, int use_masks, hipStream_t s);
+template <int D, typename R>
+void launch_init_pn(const LaunchSet& L, hipStream_t s);
 
 }  // namespace gclsph

@@ -23,6 +23,7 @@
 // summed in FP64, Float64 positions and Δt).
 #include <type_traits>
 
+#include "blockscan.hpp"
 #include "sph.hpp"
 
 namespace gclsph {
@@ -175,17 +176,86 @@ __device__ __forceinline__ void membership_ptrs(const SphBufs<R>& B, const StepF
   for (int k = 0; k < 3; ++k) XM[k] = cur ? B.X[p][k < D ? k : 1] : B.XRB[k < D ? k : 1];
 }
 
-// Particle index of this thread with an XCD-aware block order.  Workgroups are
-// dealt to the 8 XCDs round robin (block b runs on XCD b % 8); renumbering so
-// that each XCD walks one contiguous range of cell-sorted particles keeps the
-// neighbourhoods shared by consecutive blocks (and by the stencil rows above
-// and below) inside one XCD's L2 instead of fetching them on every XCD.
-__device__ __forceinline__ int64_t xcd_particle_index() {
+// Logical number of this block with an XCD-aware order: workgroups are dealt
+// to the 8 XCDs round robin (block b runs on XCD b % 8), and renumbering so
+// that each XCD walks one contiguous range of logical blocks (cell-sorted
+// particles) keeps the neighbourhoods shared by consecutive blocks, and by the
+// stencil rows above and below, inside one XCD's L2.
+__device__ __forceinline__ unsigned xcd_block() {
   const unsigned G = gridDim.x, b = blockIdx.x;
   const unsigned q = G / 8, r = G % 8, xcd = b % 8, i = b / 8;
-  const unsigned L = xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
-  return int64_t(L) * blockDim.x + threadIdx.x;
+  return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
 }
+
+// Where a kernel finds its problem.  One problem: the descriptor is a kernel
+// argument (constant memory the compiler may reload at will, which keeps
+// register pressure as low as plain arguments).  Several problems: an array of
+// descriptors in device memory and maps from blocks to (problem, block of the
+// problem) or (problem, cell tile).
+template <typename R>
+struct OneScene {
+  SceneDev<R> d;
+};
+template <typename R>
+struct ManyScenes {
+  const SceneDev<R>* __restrict__ S;
+  const int2* __restrict__ pmap;
+  const int2* __restrict__ cmap;
+};
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& particle_scene(const OneScene<R>& s, int2& m) {
+  m = make_int2(0, int(xcd_block()));
+  return s.d;
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& particle_scene(const ManyScenes<R>& s, int2& m) {
+  m = s.pmap[xcd_block()];
+  return s.S[m.x];
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& tile_scene(const OneScene<R>& s, int2& m) {
+  m = make_int2(0, int(blockIdx.x));
+  return s.d;
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& tile_scene(const ManyScenes<R>& s, int2& m) {
+  m = s.cmap[blockIdx.x];
+  return s.S[m.x];
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& scene_at(const OneScene<R>& s, unsigned) {
+  return s.d;
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& scene_at(const ManyScenes<R>& s, unsigned i) {
+  return s.S[i];
+}
+
+// Per-step switches of a problem, from its per-call switches and control block.
+template <typename R>
+__device__ __forceinline__ StepFlags step_flags(const SceneDev<R>& sd) {
+  StepFlags F = sd.F;
+  const StepCtl* c = sd.B.ctl;
+  const int it = c->iter;
+  F.synced_prev = c->synced;
+  F.cspm_fire = F.cspmn > 0 && it % F.cspmn == 0;
+  F.corr_on = F.dpc_on || F.xsph_on || F.cspm_fire;
+  F.want_out = it == c->nsteps;
+  return F;
+}
+
+// Binds the problem of this block: B, C, G, F and the particle index a.
+#define GCLSPH_SCENE(ss)                                       \
+  int2 m;                                                      \
+  const SceneDev<R>& sd = particle_scene<R>(ss, m);            \
+  const SphBufs<R>& B = sd.B;                                  \
+  const SphConst& C = sd.C;                                    \
+  const CellGeom& G = sd.G;                                    \
+  const StepFlags F = step_flags(sd);                          \
+  const int64_t a = int64_t(m.y) * kBlock + threadIdx.x;       \
+  (void)C;                                                     \
+  (void)G;                                                     \
+  (void)F
 
 // Member bits of one window [lo, hi) (hi - lo <= 32) of a stencil row: bit j is
 // set when candidate lo + j is a partner of particle a, i.e. b != a and
@@ -251,12 +321,13 @@ __device__ __forceinline__ void row_range(const CellGeom& g, const int32_t* __re
 // stored, in row order.  Entry e of particle a is NE[e * n + a]; NC[a] is the
 // entry count, or -1 when the particle needs more than necap entries (its
 // partners are then found by scanning the stencil).
-template <int D, typename R>
-__global__ __launch_bounds__(256) void k_nmask(SphBufs<R> B, CellGeom G, double dist2) {
+template <int D, typename R, typename SS>
+__global__ __launch_bounds__(256) void k_nmask(SS ss) {
+  GCLSPH_SCENE(ss);
   const StepCtl* ctl = B.ctl;
   if (!ctl->rebuilt) return;
-  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
+  const double dist2 = C.dist2;
   const int p = ctl->parity;  // the rebuild positions are the current ones
   const double* XM[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
   const double xa0 = XM[0][a], xa1 = XM[1][a], xa2 = D == 3 ? XM[2][a] : 0.0;
@@ -389,7 +460,7 @@ constexpr int kRedSlots = 64;
 // on its bit pattern (non-negative doubles order like their bits, NaN above
 // +Inf); spreading blocks over many slots avoids one hot address.  Every
 // thread of the block must call it.
-__device__ __forceinline__ void block_max(double v, bool active, double* slots) {
+__device__ __forceinline__ void block_max(double v, bool active, double* slots, int blk) {
   __shared__ double wm[kBlock / 64];
   double m = wave_max(active ? v : 0.0);
   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
@@ -398,7 +469,7 @@ __device__ __forceinline__ void block_max(double v, bool active, double* slots) 
     double r = wm[0];
 #pragma unroll
     for (int w = 1; w < kBlock / 64; ++w) r = nanmax(r, wm[w]);
-    atomic_max_nonneg(reinterpret_cast<unsigned long long*>(slots) + (blockIdx.x % kRedSlots), r);
+    atomic_max_nonneg(reinterpret_cast<unsigned long long*>(slots) + (blk % kRedSlots), r);
   }
   __syncthreads();
 }
@@ -406,8 +477,13 @@ __device__ __forceinline__ void block_max(double v, bool active, double* slots) 
 
 // ------------------------------------------------------------------ kernels
 // Runs after the (conditional) rebuild kernels: commits the reorder.
-__global__ void k_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+template <typename R, typename SS>
+__global__ void k_post_rebuild(SS ss) {
+  if (threadIdx.x != 0) return;
+  const SceneDev<R>& sd = scene_at<R>(ss, blockIdx.x);
+  StepCtl* ctl = sd.B.ctl;
+  const int32_t* maxcell = sd.maxcell;
+  const int write_xrb = sd.F.write_xrb;
   if (ctl->rebuild) {
     ctl->parity ^= 1;
     ctl->pparity ^= 1;
@@ -423,11 +499,12 @@ __global__ void k_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_x
 }
 
 // New sorted position s takes the state of old position perm[s].
-template <int D, typename R>
-__global__ void k_reorder(SphBufs<R> B, SphConst C, StepFlags F) {
+template <int D, typename R, typename SS>
+__global__ void k_reorder(SS ss) {
+  GCLSPH_SCENE(ss);
   const StepCtl* ctl = B.ctl;
   if (ctl->rebuild == 0) return;
-  int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t s = a;
   if (s >= B.n) return;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, pq = pp ^ 1;
   const int32_t src = B.perm[s];
@@ -453,13 +530,13 @@ __global__ void k_reorder(SphBufs<R> B, SphConst C, StepFlags F) {
 }
 
 // ---- pass 1 ---------------------------------------------------------------
-template <int D, typename R>
-__global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+template <int D, typename R, typename SS>
+__global__ __launch_bounds__(256) void k_pass1(SS ss) {
 #pragma clang fp contract(off)
+  GCLSPH_SCENE(ss);
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
-  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const double* X[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
   const R* V[3] = {B.V[p][0], B.V[p][1], B.V[p][D == 3 ? 2 : 1]};
@@ -723,13 +800,13 @@ __global__ __launch_bounds__(256) void k_pass1(SphBufs<R> B, SphConst C, CellGeo
 }
 
 // ---- pass 2 + symplectic update ------------------------------------------
-template <int D, typename R>
-__global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+template <int D, typename R, typename SS>
+__global__ __launch_bounds__(256) void k_pass2(SS ss) {
 #pragma clang fp contract(off)
+  GCLSPH_SCENE(ss);
   StepCtl* ctl = B.ctl;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
-  const int64_t a = xcd_particle_index();
   const bool active = a < B.n;
   double red_cdx = 0.0, red_acc = 0.0, red_visc = 0.0;
   if (active) {
@@ -1023,22 +1100,23 @@ __global__ __launch_bounds__(256) void k_pass2(SphBufs<R> B, SphConst C, CellGeo
       B.PBUF[a] = Pha;
     }
   }
-  block_max(red_cdx, active, B.RED);
+  block_max(red_cdx, active, B.RED, m.y);
   if (F.ts) {
-    block_max(red_acc, active, B.RED + kRedSlots);
-    if (F.visc_in_pass2) block_max(red_visc, active, B.RED + 2 * kRedSlots);
+    block_max(red_acc, active, B.RED + kRedSlots, m.y);
+    if (F.visc_in_pass2) block_max(red_visc, active, B.RED + 2 * kRedSlots, m.y);
   }
 }
 
 // ---- corrections after the update (src/sphproblem.jl:303-322) --------------
 // dpcreg! (src/gpukernels_sph.jl:1500-1553): ∑Δv_dpc into DV; P = EOS(ρ^{n+1}).
-template <int D, typename R>
-__global__ __launch_bounds__(256) void k_dpc_sum(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+template <int D, typename R, typename SS>
+__global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
 #pragma clang fp contract(off)
+  GCLSPH_SCENE(ss);
+  if (!F.dpc_on) return;
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
-  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const double* X[3] = {B.X[q][0], B.X[q][1], B.X[q][D == 3 ? 2 : 1]};
   const R* V[3] = {B.V[q][0], B.V[q][1], B.V[q][D == 3 ? 2 : 1]};
@@ -1089,13 +1167,14 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SphBufs<R> B, SphConst C, CellG
 }
 
 // update_dpcreg! (src/gpukernels_sph.jl:1617-1632)
-template <int D, typename R>
-__global__ void k_dpc_apply(SphBufs<R> B) {
+template <int D, typename R, typename SS>
+__global__ void k_dpc_apply(SS ss) {
 #pragma clang fp contract(off)
+  GCLSPH_SCENE(ss);
+  if (!F.dpc_on) return;
   const StepCtl* ctl = B.ctl;
   const int q = ctl->parity ^ 1, pp = ctl->pparity;
   const double dt = ctl->dt;
-  const int64_t a = xcd_particle_index();
   if (a >= B.n || B.PT[pp][a] <= 1) return;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
@@ -1106,13 +1185,14 @@ __global__ void k_dpc_apply(SphBufs<R> B) {
 }
 
 // xsphcorr! + update_xsphcorr! (src/gpukernels_sph.jl:1757-1838): W at xⁿ.
-template <int D, typename R>
-__global__ __launch_bounds__(256) void k_xsph(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+template <int D, typename R, typename SS>
+__global__ __launch_bounds__(256) void k_xsph(SS ss) {
 #pragma clang fp contract(off)
+  GCLSPH_SCENE(ss);
+  if (!F.xsph_on) return;
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
-  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const int32_t* PT = B.PT[pp];
   const int32_t pta = PT[a];
@@ -1142,12 +1222,13 @@ __global__ __launch_bounds__(256) void k_xsph(SphBufs<R> B, SphConst C, CellGeom
 }
 
 // cspmcorr! (src/gpukernels_sph.jl:1693-1719): ρ = Σ m0 W / Σ m0 W / ρ_b, W at xⁿ.
-template <int D, typename R>
-__global__ __launch_bounds__(256) void k_cspm_sum(SphBufs<R> B, SphConst C, CellGeom G, StepFlags F) {
+template <int D, typename R, typename SS>
+__global__ __launch_bounds__(256) void k_cspm_sum(SS ss) {
 #pragma clang fp contract(off)
+  GCLSPH_SCENE(ss);
+  if (!F.cspm_fire) return;
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
-  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const int32_t* PT = B.PT[pp];
   const int32_t pta = PT[a];
@@ -1172,11 +1253,12 @@ __global__ __launch_bounds__(256) void k_cspm_sum(SphBufs<R> B, SphConst C, Cell
   B.TMP[a] = s1 / s2;
 }
 
-template <typename R>
-__global__ void k_cspm_apply(SphBufs<R> B, SphConst C) {
+template <typename R, typename SS>
+__global__ void k_cspm_apply(SS ss) {
+  GCLSPH_SCENE(ss);
+  if (!F.cspm_fire) return;
   const StepCtl* ctl = B.ctl;
   const int q = ctl->parity ^ 1, pp = ctl->pparity;
-  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   double nr = B.TMP[a];
   if (nr != nr) return;
@@ -1192,11 +1274,11 @@ __global__ void k_cspm_apply(SphBufs<R> B, SphConst C) {
 }
 
 // PN = EOS(current density) for every particle (after construction).
-template <typename R>
-__global__ void k_init_pn(SphBufs<R> B, SphConst C) {
+template <typename R, typename SS>
+__global__ void k_init_pn(SS ss) {
+  GCLSPH_SCENE(ss);
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, pp = ctl->pparity;
-  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const R rv = B.RHO[p][a];
   const int32_t pt = B.PT[pp][a];
@@ -1205,11 +1287,12 @@ __global__ void k_init_pn(SphBufs<R> B, SphConst C) {
 }
 
 // Δt viscous rate over the corrected state (src/gpukernels_sph.jl:1325-1334).
-template <int D, typename R>
-__global__ void k_dt_visc(SphBufs<R> B, SphConst C) {
+template <int D, typename R, typename SS>
+__global__ void k_dt_visc(SS ss) {
+  GCLSPH_SCENE(ss);
+  if (!(F.ts && !F.visc_in_pass2)) return;  // the whole block belongs to one problem
   StepCtl* ctl = B.ctl;
   const int q = ctl->parity ^ 1;
-  const int64_t a = xcd_particle_index();
   const bool active = a < B.n;
   double v = 0.0;
   if (active) {
@@ -1221,7 +1304,7 @@ __global__ void k_dt_visc(SphBufs<R> B, SphConst C) {
     }
     v = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
   }
-  block_max(v, active, B.RED + 2 * kRedSlots);
+  block_max(v, active, B.RED + 2 * kRedSlots, m.y);
 }
 
 __device__ __forceinline__ double jl_min(double a, double b) { return (a != a || b != b) ? a + b : (a < b ? a : b); }
@@ -1229,8 +1312,16 @@ __device__ __forceinline__ double jl_max(double a, double b) { return (a != a ||
 
 // End of step: commits the new state, rebuild flag, time and Δt
 // (src/sphproblem.jl:325-335, src/gpukernels_sph.jl:1293-1323).
-__global__ void k_finalize(StepCtl* ctl, double* __restrict__ red, SphConst C, int ts, int corr_on) {
-  // one wave: lane l reads slot l of each reduction and clears it for the next step
+template <typename R, typename SS>
+__global__ void k_finalize(SS ss) {
+  // one wave per problem: lane l reads slot l of each reduction and clears it
+  // for the next step
+  const SceneDev<R>& sd = scene_at<R>(ss, blockIdx.x);
+  const StepFlags F = step_flags(sd);
+  const SphConst& C = sd.C;
+  StepCtl* ctl = sd.B.ctl;
+  double* red = sd.B.RED;
+  const int ts = F.ts, corr_on = F.corr_on;
   const int l = threadIdx.x;
   double tot[3];
   for (int k = 0; k < 3; ++k) {
@@ -1258,15 +1349,16 @@ __global__ void k_finalize(StepCtl* ctl, double* __restrict__ red, SphConst C, i
   ctl->parity ^= 1;
   ctl->synced = corr_on ? 0 : 1;
   ctl->xrb_cur = 0;
+  ctl->iter += 1;
 }
 
 // sph∑W! + sph∑∇W! over the current positions (src/gpukernels_sph.jl:139-156, :332-359).
-template <int D, typename R>
-__global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom G, int xm_mode, int use_masks) {
+template <int D, typename R, typename SS>
+__global__ __launch_bounds__(256) void k_diag(SS ss, int xm_mode, int use_masks) {
 #pragma clang fp contract(off)
+  GCLSPH_SCENE(ss);
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, pp = ctl->pparity;
-  const int64_t a = xcd_particle_index();
   if (a >= B.n) return;
   const double* X[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
   // membership positions: 0 = stored rebuild positions, 1 = the step's start
@@ -1296,74 +1388,244 @@ __global__ __launch_bounds__(256) void k_diag(SphBufs<R> B, SphConst C, CellGeom
   for (int d = 0; d < D; ++d) B.SUMGW[d][a] = sg[d];
 }
 
-// ------------------------------------------------------------ host launchers
-template <int D, typename R>
-void launch_reorder(const SphBufs<R>& B, const SphConst& C, const StepFlags& F, hipStream_t s) {
-  k_reorder<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, F);
-}
-template <typename R>
-void launch_init_pn(const SphBufs<R>& B, const SphConst& C, hipStream_t s) {
-  k_init_pn<R><<<grid_for(B.n), kBlock, 0, s>>>(B, C);
-}
-template void launch_init_pn<double>(const SphBufs<double>&, const SphConst&, hipStream_t);
-template void launch_init_pn<float>(const SphBufs<float>&, const SphConst&, hipStream_t);
-template <int D, typename R>
-void launch_pass1(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
-  k_pass1<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
-}
-template <int D, typename R>
-void launch_pass2(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
-  k_pass2<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
-}
-template <int D, typename R>
-void launch_dpc(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
-  k_dpc_sum<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
-  k_dpc_apply<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B);
-}
-template <int D, typename R>
-void launch_xsph(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
-  k_xsph<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);
-}
-template <int D, typename R>
-void launch_cspm(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, const StepFlags& F, hipStream_t s) {
-  k_cspm_sum<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, F);  k_cspm_apply<R><<<grid_for(B.n), kBlock, 0, s>>>(B, C);
-}
-template <int D, typename R>
-void launch_dt_visc(const SphBufs<R>& B, const SphConst& C, hipStream_t s) {
-  k_dt_visc<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C);
-}
-template <int D, typename R>
-void launch_diag(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int xm_mode, int use_masks,
-                 hipStream_t s) {
-  k_diag<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, C, G, xm_mode, use_masks);
-}
-template <int D, typename R>
-void launch_nmask(const SphBufs<R>& B, const CellGeom& G, double dist2, hipStream_t s) {
-  k_nmask<D, R><<<grid_for(B.n), kBlock, 0, s>>>(B, G, dist2);
-}
-void launch_post_rebuild(StepCtl* ctl, const int32_t* maxcell, int write_xrb, hipStream_t s) {
-  k_post_rebuild<<<1, 64, 0, s>>>(ctl, maxcell, write_xrb);
-}
-void launch_finalize(StepCtl* ctl, const double* red, int nblk, const SphConst& C, int ts, int corr_on,
-                     hipStream_t s) {
-  k_finalize<<<1, kRedSlots, 0, s>>>(ctl, const_cast<double*>(red), C, ts, corr_on);
+// ------------------------------------------------- rebuild: counting sort
+// The problem's cell-sorted order is rebuilt by a counting sort over its cells
+// (src/gpukernels_cls.jl:29-101 bins particles with atomics into a dense table):
+// histogram with per-particle arrival slots, exclusive scan into cell starts,
+// scatter, and a per-cell rank by input index (the previous sorted position),
+// which makes the order deterministic.  Every kernel returns at once for a
+// problem whose control block does not ask for a rebuild.
+
+// cnt[0..ncells] = 0 and maxcell = 0 (cell tiles).
+template <typename R, typename SS>
+__global__ void kb_clear_cells(SS ss) {
+  int2 m;
+  const SceneDev<R>& sd = tile_scene<R>(ss, m);
+  if (sd.B.ctl->rebuild == 0) return;
+  const int64_t base = int64_t(m.y) * kTile;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + int64_t(k) * kScanThreads + threadIdx.x;
+    if (i <= sd.ncells) sd.cnt[i] = 0;
+  }
+  if (m.y == 0 && threadIdx.x == 0) *sd.maxcell = 0;
 }
 
-#define GCLSPH_INSTANTIATE(D, R)                                                                               \
-  template void launch_reorder<D, R>(const SphBufs<R>&, const SphConst&, const StepFlags&, hipStream_t);                        \
-  template void launch_pass1<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,      \
-                                   hipStream_t);                                                               \
-  template void launch_pass2<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,      \
-                                   hipStream_t);                                                               \
-  template void launch_dpc<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,        \
-                                 hipStream_t);                                                                 \
-  template void launch_xsph<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,       \
-                                  hipStream_t);                                                                \
-  template void launch_cspm<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, const StepFlags&,       \
-                                  hipStream_t);                                                                \
-  template void launch_dt_visc<D, R>(const SphBufs<R>&, const SphConst&, hipStream_t);                         \
-  template void launch_diag<D, R>(const SphBufs<R>&, const SphConst&, const CellGeom&, int, int, hipStream_t);  \
-  template void launch_nmask<D, R>(const SphBufs<R>&, const CellGeom&, double, hipStream_t);
+// Cell of every particle (current positions) and its arrival slot.
+template <int D, typename R, typename SS>
+__global__ void kb_cellmap(SS ss) {
+  GCLSPH_SCENE(ss);
+  const StepCtl* ctl = B.ctl;
+  if (ctl->rebuild == 0 || a >= B.n) return;
+  const int p = ctl->parity;
+  int32_t c[3] = {1, 1, 1};
+#pragma unroll
+  for (int k = 0; k < D; ++k) c[k] = cell_coord<double>(B.X[p][k][a], G, k, 0);
+  const int64_t lin = int64_t(c[0] - 1) + G.grid[0] * (int64_t(c[1] - 1) + G.grid[1] * int64_t(c[2] - 1));
+  const int32_t key = int32_t(lin);
+  sd.key[a] = key;
+  sd.slot[a] = wave_aggregated_slot(sd.cnt, key);
+}
+
+// Tile sums (and tile maxima, stored after the sums) of the cell counts.
+template <typename R, typename SS>
+__global__ void kb_tile_sums(SS ss) {
+  int2 m;
+  const SceneDev<R>& sd = tile_scene<R>(ss, m);
+  if (sd.B.ctl->rebuild == 0) return;
+  const int64_t n = sd.ncells, nt = (n + kTile - 1) / kTile;
+  if (m.y >= nt) return;
+  const int64_t base = int64_t(m.y) * kTile;
+  int64_t s = 0;
+  int32_t mx = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + int64_t(k) * kScanThreads + threadIdx.x;
+    if (i < n) {
+      const int32_t v = sd.cnt[i];
+      s += v;
+      mx = v > mx ? v : mx;
+    }
+  }
+  int64_t tot;
+  block_exclusive_scan<int64_t>(s, &tot);
+  if (threadIdx.x == 0) sd.scan_tmp[m.y] = tot;
+  __shared__ int32_t wmx[kScanThreads / 64];
+  for (int o = 32; o > 0; o >>= 1) {
+    const int32_t t = __shfl_xor(mx, o, 64);
+    mx = t > mx ? t : mx;
+  }
+  if ((threadIdx.x & 63) == 0) wmx[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t v = wmx[0];
+    for (int w = 1; w < kScanThreads / 64; ++w) v = wmx[w] > v ? wmx[w] : v;
+    reinterpret_cast<int32_t*>(sd.scan_tmp + nt + 2)[m.y] = v;
+  }
+}
+
+// One block per problem: exclusive scan of the tile sums in place (total at
+// [nt]) and the maximum cell count.
+template <typename R, typename SS>
+__global__ void kb_scan_tile_sums(SS ss) {
+  const SceneDev<R>& sd = scene_at<R>(ss, blockIdx.x);
+  if (sd.B.ctl->rebuild == 0) return;
+  int64_t* tsum = sd.scan_tmp;
+  const int64_t nt = (sd.ncells + kTile - 1) / kTile;
+  const int32_t* tm = reinterpret_cast<const int32_t*>(tsum + nt + 2);
+  int32_t mx = 0;
+  for (int64_t i = threadIdx.x; i < nt; i += kScanThreads) mx = tm[i] > mx ? tm[i] : mx;
+  for (int o = 32; o > 0; o >>= 1) {
+    const int32_t t = __shfl_xor(mx, o, 64);
+    mx = t > mx ? t : mx;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(sd.maxcell, mx);
+  const int64_t per = (nt + kScanThreads - 1) / kScanThreads;
+  const int64_t b = int64_t(threadIdx.x) * per, e = b + per < nt ? b + per : nt;
+  int64_t s = 0;
+  for (int64_t i = b; i < e; ++i) s += tsum[i];
+  int64_t tot;
+  int64_t pre = block_exclusive_scan<int64_t>(s, &tot);
+  for (int64_t i = b; i < e; ++i) {
+    const int64_t v = tsum[i];
+    tsum[i] = pre;
+    pre += v;
+  }
+  if (threadIdx.x == 0) tsum[nt] = tot;
+}
+
+// Cell starts: tile scans plus the tile prefixes; start[ncells] = n.
+template <typename R, typename SS>
+__global__ void kb_tile_scan(SS ss) {
+  int2 m;
+  const SceneDev<R>& sd = tile_scene<R>(ss, m);
+  if (sd.B.ctl->rebuild == 0) return;
+  const int64_t n = sd.ncells, nt = (n + kTile - 1) / kTile;
+  if (m.y >= nt) return;
+  const int64_t base = int64_t(m.y) * kTile + int64_t(threadIdx.x) * kScanItems;
+  int32_t v[kScanItems];
+  int64_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    v[k] = i < n ? sd.cnt[i] : 0;
+    s += v[k];
+  }
+  int64_t tot;
+  int64_t pre = block_exclusive_scan<int64_t>(s, &tot) + sd.scan_tmp[m.y];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k;
+    if (i < n) sd.start[i] = int32_t(pre);
+    pre += v[k];
+  }
+  if (m.y == 0 && threadIdx.x == 0) sd.start[n] = int32_t(sd.scan_tmp[nt]);
+}
+
+template <int D, typename R, typename SS>
+__global__ void kb_scatter(SS ss) {
+  GCLSPH_SCENE(ss);
+  if (B.ctl->rebuild == 0 || a >= B.n) return;
+  sd.tmp[sd.start[sd.key[a]] + sd.slot[a]] = int32_t(a);
+}
+
+// Rank inside the cell by input index: the final order is (cell, input).
+template <int D, typename R, typename SS>
+__global__ void kb_rank(SS ss) {
+  GCLSPH_SCENE(ss);
+  if (B.ctl->rebuild == 0 || a >= B.n) return;
+  const int32_t i = sd.tmp[a];
+  const int32_t c = sd.key[i];
+  const int32_t b = sd.start[c], e = sd.start[c + 1];
+  int32_t r = 0;
+  for (int32_t k = b; k < e; ++k) r += sd.tmp[k] < i;
+  sd.perm[b + r] = i;
+  sd.skey[b + r] = c;
+}
+
+// ------------------------------------------------------------ host launchers
+namespace {
+// Calls f with the scene source of L: the descriptor by value for one problem,
+// the device descriptor array and maps for several.
+template <typename R, typename Fn>
+void with_scenes(const LaunchSet& L, Fn&& f) {
+  if (L.one) f(OneScene<R>{*static_cast<const SceneDev<R>*>(L.one)});
+  else f(ManyScenes<R>{static_cast<const SceneDev<R>*>(L.scenes), L.pmap, L.cmap});
+}
+}  // namespace
+
+template <int D, typename R>
+void launch_rebuild(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) {
+    using SS = decltype(ss);
+    kb_clear_cells<R, SS><<<L.ctiles, kScanThreads, 0, s>>>(ss);
+    kb_cellmap<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
+    kb_tile_sums<R, SS><<<L.ctiles, kScanThreads, 0, s>>>(ss);
+    kb_scan_tile_sums<R, SS><<<L.count, kScanThreads, 0, s>>>(ss);
+    kb_tile_scan<R, SS><<<L.ctiles, kScanThreads, 0, s>>>(ss);
+    kb_scatter<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
+    kb_rank<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
+    k_reorder<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
+    k_post_rebuild<R, SS><<<L.count, 64, 0, s>>>(ss);
+    k_nmask<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
+  });
+}
+template <int D, typename R>
+void launch_init_pn(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) { k_init_pn<R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss); });
+}
+template <int D, typename R>
+void launch_pass1(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) { k_pass1<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss); });
+}
+template <int D, typename R>
+void launch_pass2(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) { k_pass2<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss); });
+}
+template <int D, typename R>
+void launch_dpc(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) {
+    k_dpc_sum<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss);
+    k_dpc_apply<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss);
+  });
+}
+template <int D, typename R>
+void launch_xsph(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) { k_xsph<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss); });
+}
+template <int D, typename R>
+void launch_cspm(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) {
+    k_cspm_sum<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss);
+    k_cspm_apply<R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss);
+  });
+}
+template <int D, typename R>
+void launch_dt_visc(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) { k_dt_visc<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss); });
+}
+template <int D, typename R>
+void launch_finalize(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) { k_finalize<R, decltype(ss)><<<L.count, kRedSlots, 0, s>>>(ss); });
+}
+template <int D, typename R>
+void launch_diag(const LaunchSet& L, int xm_mode, int use_masks, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) {
+    k_diag<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss, xm_mode, use_masks);
+  });
+}
+
+#define GCLSPH_INSTANTIATE(D, R)                                                         \
+  template void launch_rebuild<D, R>(const LaunchSet&, hipStream_t);                     \
+  template void launch_init_pn<D, R>(const LaunchSet&, hipStream_t);                     \
+  template void launch_pass1<D, R>(const LaunchSet&, hipStream_t);                       \
+  template void launch_pass2<D, R>(const LaunchSet&, hipStream_t);                       \
+  template void launch_dpc<D, R>(const LaunchSet&, hipStream_t);                         \
+  template void launch_xsph<D, R>(const LaunchSet&, hipStream_t);                        \
+  template void launch_cspm<D, R>(const LaunchSet&, hipStream_t);                        \
+  template void launch_dt_visc<D, R>(const LaunchSet&, hipStream_t);                     \
+  template void launch_finalize<D, R>(const LaunchSet&, hipStream_t);                    \
+  template void launch_diag<D, R>(const LaunchSet&, int, int, hipStream_t);
 
 GCLSPH_INSTANTIATE(2, double)
 GCLSPH_INSTANTIATE(3, double)

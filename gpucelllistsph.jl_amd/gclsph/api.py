@@ -321,6 +321,51 @@ def stepsolve_(prob: SPHProblem, n: int = 1, timestepping: bool = False,
     return int(upd.value), float(mx.value)
 
 
+class SPHBatch:
+    """Independent problems of one dimension and precision stepped together:
+    every stage of the step is one launch over all of them (the batched
+    ensemble).  Each problem keeps its own Δt, rebuilds, time and corrections
+    and ends as if stepped alone; read results through the problems.  While a
+    batch exists its problems cannot be destroyed (the batch keeps them alive)."""
+
+    def __init__(self, problems: Sequence[SPHProblem]):
+        probs = list(problems)
+        arr = (C.c_void_p * len(probs))(*[p._h for p in probs])
+        hh = C.c_void_p()
+        check(lib().sph_batch_create(arr, len(probs), C.byref(hh)))
+        self._h = hh
+        self.problems = probs
+
+    def __len__(self):
+        return len(self.problems)
+
+    def __del__(self):
+        h = self.__dict__.get("_h")
+        if h:
+            lib().sph_batch_destroy(h)
+            self._h = None
+
+
+def batch_stepsolve_(batch: SPHBatch, n: int = 1, timestepping: bool = False,
+                     timelims: Optional[Sequence[Tuple[float, float]]] = None) -> Tuple[np.ndarray, np.ndarray]:
+    """stepsolve!(prob, n; timestepping, timelims) for every problem of the batch
+    in one launch per stage.  timelims: one (min, max) pair per problem, or None
+    for each problem's default.  Returns (updaten, maxcΔxout) per problem."""
+    k = len(batch)
+    upd = np.zeros(k, dtype=np.int64)
+    mx = np.zeros(k, dtype=np.float64)
+    t0 = t1 = None
+    if timelims is not None:
+        t0 = np.ascontiguousarray([float(t[0]) for t in timelims], dtype=np.float64)
+        t1 = np.ascontiguousarray([float(t[1]) for t in timelims], dtype=np.float64)
+        if len(t0) != k:
+            raise ValueError("one timelims pair per problem")
+    check(lib().sph_batch_stepsolve(batch._h, int(n), int(bool(timestepping)), None if t0 is None else _dp(t0),
+                                    None if t1 is None else _dp(t1), upd.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    _dp(mx)))
+    return upd, mx
+
+
 def timesolve_(prob: SPHProblem, batch: int = 10, timeframe: float = 1.0, writetime: float = 0, path=None,
                pvc: bool = False, vtkvars=("Acceleration", "Velocity", "Pressure"), timestepping: bool = False,
                timelims: Optional[Tuple[float, float]] = None, anim: bool = False, verbose: bool = True):

@@ -40,6 +40,7 @@ extern "C" {
 
 typedef struct gcl_s* gcl_t;
 typedef struct sph_s* sph_t;
+typedef struct sph_batch_s* sph_batch_t;
 
 /* Message of the last failed call on this thread ("" if none). */
 const char* gclsph_last_error(void);
@@ -137,6 +138,22 @@ enum {
   SPH_FIELD_PTYPE = 12 /* get_ptype :388 (as doubles) */
 };
 int sph_get_field(sph_t h, int field, double* out);
+
+/* Batched ensemble (no counterpart in the reference: the batching pattern for
+ * many independent scenes of uneven size).  A batch steps `count` existing
+ * problems of one dimension and precision together: every stage of the step
+ * is one launch over all of them, while each problem keeps its own Δt,
+ * rebuild schedule, time and corrections, so every problem ends exactly as if
+ * stepped alone with sph_stepsolve.  The problems stay usable on their own
+ * (getters, parameters, single stepping); a problem cannot be destroyed while
+ * a batch holds it. */
+int sph_batch_create(const sph_t* problems, int32_t count, sph_batch_t* out);
+int sph_batch_destroy(sph_batch_t b);
+/* n steps of every problem (stepsolve! per problem, src/sphproblem.jl:207-345).
+ * tl0 / tl1: per-problem Δt limits, or NULL for each problem's default
+ * (sqrt(eps), CFL·H/(3c0)).  updaten / maxcdx: count outputs. */
+int sph_batch_stepsolve(sph_batch_t b, int64_t n, int timestepping, const double* tl0, const double* tl1,
+                        int64_t* updaten, double* maxcdx);
 
 /* Per-stage GPU timing over the calls since the last reset, measured with HIP
  * events on the problem's stream (enabled by sph_set_timing(h, 1)).  Writes up
