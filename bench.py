@@ -63,6 +63,37 @@ def run_gpu(scene, precision: str, steps: int, warmup: int):
     return {"seconds": t1 - t0, "updaten": upd, "timing": timing, "dt": prob.dt}
 
 
+def run_ensemble(steps: int, warmup: int):
+    """The batched ensemble: 64 seeded 2D dam breaks of uneven size (FP32-split),
+    all stepped by one batch (one launch per stage, a captured step graph),
+    against the same problems stepped one by one."""
+    scs = S.config4()
+    probs = []
+    for sc in scs:
+        cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        probs.append(G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, precision="f32split"))
+    total = sum(sc.n for sc in scs)
+    batch = G.SPHBatch(probs)
+    G.batch_stepsolve_(batch, warmup, timestepping=True)
+    t0 = time.perf_counter()
+    G.batch_stepsolve_(batch, steps, timestepping=True)
+    tb = time.perf_counter() - t0
+    for p in probs:
+        G.stepsolve_(p, warmup, timestepping=True)
+    t0 = time.perf_counter()
+    for p in probs:
+        G.stepsolve_(p, steps, timestepping=True)
+    t1 = time.perf_counter() - t0
+    del batch
+    return {"scenes": len(scs), "particles": total, "min_particles": min(sc.n for sc in scs),
+            "max_particles": max(sc.n for sc in scs), "steps": steps,
+            "batched_particle_steps_per_s": total * steps / tb, "batched_ms_per_step": tb / steps * 1e3,
+            "one_by_one_particle_steps_per_s": total * steps / t1, "one_by_one_ms_per_step": t1 / steps * 1e3,
+            "speedup": t1 / tb,
+            "workload": "64 2D dam breaks, dx in {0.005,0.01,0.02}, 0.5-1.5 x 1.0-2.0 m columns, 4x3 m tanks, "
+                        "FP32-split, adaptive CFL"}
+
+
 def read_profile(path: str) -> dict:
     """`key: value` lines of the committed rocprofv3 PMC summary of the dominant
     kernel: HBM bytes per dispatch (FETCH_SIZE / WRITE_SIZE scaled by the
@@ -105,6 +136,7 @@ def main():
     ap.add_argument("--dx", type=float, default=0.002)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pass2_f32split_traffic.txt"))
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="copy a buffer of this many GiB three times and exit (HBM counter calibration)")
@@ -179,6 +211,8 @@ def main():
         out["f64"] = {"value": v64, "ms_per_step": res64["seconds"] / args.steps * 1e3,
                       "stage_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in res64["timing"].items() if v[1] > 0}}
         out["f32split_over_f64"] = value / v64
+    if not args.no_ensemble:
+        out["ensemble"] = run_ensemble(args.steps, min(args.warmup, 3))
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene)
     print(json.dumps(out))

@@ -660,6 +660,9 @@ void LaunchSet::build(const int64_t* n, const int64_t* ncells, unsigned cnt, siz
 }
 
 void LaunchSet::release() {
+  if (step_graph) (void)hipGraphExecDestroy(step_graph);
+  step_graph = nullptr;
+  graph_key.clear();
   if (scenes) (void)hipFree(scenes);
   if (pmap) (void)hipFree(pmap);
   if (cmap) (void)hipFree(cmap);
@@ -708,43 +711,90 @@ void run_steps(sph_s* const* P, unsigned count, LaunchSet& L, hipStream_t st, in
   if (count == 1) L.one = descs.data();
   else GCL_HIP(hipMemcpyAsync(L.scenes, descs.data(), sizeof(SceneDev<R>) * count, hipMemcpyHostToDevice, st));
   sph_s* timer = (count == 1 && P[0]->timing) ? P[0] : nullptr;
-  auto mark_begin = [&]() { return timer ? timer->mark_begin(st) : hipEvent_t(nullptr); };
-  auto mark_end = [&](int stage, hipEvent_t e) {
-    if (timer) timer->mark_end(stage, e, st);
-  };
-  for (int64_t it = 1; it <= nsteps; ++it) {
-    bool any_cspm = false;
-    for (unsigned i = 0; i < count; ++i) any_cspm |= flags[i].cspmn > 0 && it % flags[i].cspmn == 0;
-    hipEvent_t e0 = mark_begin();
+  bool any_cspmn = false;
+  for (unsigned i = 0; i < count; ++i) any_cspmn |= flags[i].cspmn > 0;
+  // one step's launches; with the corrections of every problem that may need
+  // them (each kernel returns at once for a problem that does not)
+  auto launch_step = [&](bool cspm) {
     launch_rebuild<D, R>(L, st);
-    GCL_HIP(hipGetLastError());
-    mark_end(0, e0);
-    e0 = mark_begin();
     launch_pass1<D, R>(L, st);
-    GCL_HIP(hipGetLastError());
-    mark_end(1, e0);
-    e0 = mark_begin();
     launch_pass2<D, R>(L, st);
-    GCL_HIP(hipGetLastError());
-    mark_end(2, e0);
-    if (any_dpc || any_xsph || any_cspm || any_dtv) {
-      e0 = mark_begin();
-      if (any_dpc) launch_dpc<D, R>(L, st);
-      if (any_xsph) launch_xsph<D, R>(L, st);
-      if (any_cspm) launch_cspm<D, R>(L, st);
-      if (any_dtv) launch_dt_visc<D, R>(L, st);
-      GCL_HIP(hipGetLastError());
-      mark_end(3, e0);
-    }
-    e0 = mark_begin();
+    if (any_dpc) launch_dpc<D, R>(L, st);
+    if (any_xsph) launch_xsph<D, R>(L, st);
+    if (cspm) launch_cspm<D, R>(L, st);
+    if (any_dtv) launch_dt_visc<D, R>(L, st);
     launch_finalize<D, R>(L, st);
     GCL_HIP(hipGetLastError());
-    mark_end(4, e0);
-    for (unsigned i = 0; i < count; ++i) {
-      const StepFlags& F = flags[i];
-      const bool cspm_fire = F.cspmn > 0 && it % F.cspmn == 0;
-      P[i]->synced_prev = !(F.dpc_on || F.xsph_on || cspm_fire);
+  };
+  const char* no_graph = std::getenv("GCLSPH_NO_GRAPH");
+  const bool graph = !timer && nsteps >= 2 && !(no_graph && no_graph[0] == '1');
+  if (graph) {
+    // The step as a captured graph, replayed nsteps times: every per-step
+    // decision lives in the control blocks, so one capture serves all steps.
+    // Kept for the next call while the launches and one-problem descriptor
+    // (baked into the graph) are unchanged.
+    std::vector<unsigned char> key = {uint8_t(any_dpc), uint8_t(any_xsph), uint8_t(any_cspmn), uint8_t(any_dtv)};
+    if (L.one) {
+      const unsigned char* d = static_cast<const unsigned char*>(L.one);
+      key.insert(key.end(), d, d + sizeof(SceneDev<R>));
     }
+    if (!L.step_graph || key != L.graph_key) {
+      if (L.step_graph) GCL_HIP(hipGraphExecDestroy(L.step_graph));
+      L.step_graph = nullptr;
+      hipGraph_t g = nullptr;
+      GCL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      launch_step(any_cspmn);
+      GCL_HIP(hipStreamEndCapture(st, &g));
+      const hipError_t e = hipGraphInstantiate(&L.step_graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      GCL_HIP(e);
+      L.graph_key = std::move(key);
+    }
+    for (int64_t it = 1; it <= nsteps; ++it) GCL_HIP(hipGraphLaunch(L.step_graph, st));
+  } else {
+    auto mark_begin = [&]() { return timer ? timer->mark_begin(st) : hipEvent_t(nullptr); };
+    auto mark_end = [&](int stage, hipEvent_t e) {
+      if (timer) timer->mark_end(stage, e, st);
+    };
+    for (int64_t it = 1; it <= nsteps; ++it) {
+      bool any_cspm = false;
+      for (unsigned i = 0; i < count; ++i) any_cspm |= flags[i].cspmn > 0 && it % flags[i].cspmn == 0;
+      if (!timer) {
+        launch_step(any_cspm);
+        continue;
+      }
+      hipEvent_t e0 = mark_begin();
+      launch_rebuild<D, R>(L, st);
+      GCL_HIP(hipGetLastError());
+      mark_end(0, e0);
+      e0 = mark_begin();
+      launch_pass1<D, R>(L, st);
+      GCL_HIP(hipGetLastError());
+      mark_end(1, e0);
+      e0 = mark_begin();
+      launch_pass2<D, R>(L, st);
+      GCL_HIP(hipGetLastError());
+      mark_end(2, e0);
+      if (any_dpc || any_xsph || any_cspm || any_dtv) {
+        e0 = mark_begin();
+        if (any_dpc) launch_dpc<D, R>(L, st);
+        if (any_xsph) launch_xsph<D, R>(L, st);
+        if (any_cspm) launch_cspm<D, R>(L, st);
+        if (any_dtv) launch_dt_visc<D, R>(L, st);
+        GCL_HIP(hipGetLastError());
+        mark_end(3, e0);
+      }
+      e0 = mark_begin();
+      launch_finalize<D, R>(L, st);
+      GCL_HIP(hipGetLastError());
+      mark_end(4, e0);
+    }
+  }
+  // host mirror of the device's half-step sync flag after the last step
+  for (unsigned i = 0; i < count && nsteps > 0; ++i) {
+    const StepFlags& F = flags[i];
+    const bool cspm_fire = F.cspmn > 0 && nsteps % F.cspmn == 0;
+    P[i]->synced_prev = !(F.dpc_on || F.xsph_on || cspm_fire);
   }
   for (unsigned i = 0; i < count; ++i) {
     sph_s* p = P[i];

@@ -1,6 +1,8 @@
 // SPH step: device state, constants and the host-side driver.
 #pragma once
 
+#include <vector>
+
 #include "celllist.hpp"
 
 namespace gclsph {
@@ -122,6 +124,9 @@ struct LaunchSet {
   int2* cmap = nullptr;
   unsigned pblocks = 0, ctiles = 0, count = 0;
   size_t scene_bytes = 0;
+  // the captured step of the last call and what it was captured for
+  hipGraphExec_t step_graph = nullptr;
+  std::vector<unsigned char> graph_key;
   // n[i] particles and ncells[i] cells of problem i
   void build(const int64_t* n, const int64_t* ncells, unsigned count, size_t scene_bytes);
   void release();

@@ -272,6 +272,19 @@ def config3(dx: float = 0.002, fluid: float = 4.0) -> Scene:
     return dam_break_2d(dx, fluid, fluid, 16.0, 8.0, name=f"dam-break-2d dx={dx} ({fluid}x{fluid} m column)")
 
 
+def config4(seed: int = 4, count: int = 64) -> list:
+    """Ensemble of 2D dam breaks drawn from a seed: dx in {0.005, 0.01, 0.02}, a column 0.5-1.5 m wide and
+    1.0-2.0 m high in a 4.0 x 3.0 m tank with 3 wall layers (1,500-120,000 particles per scene)."""
+    u = [uniform01(seed, count, k) for k in range(3)]
+    out = []
+    for i in range(count):
+        dx = (0.005, 0.01, 0.02)[min(int(u[0][i] * 3), 2)]
+        w = 0.5 + float(u[1][i])
+        h = 1.0 + float(u[2][i])
+        out.append(dam_break_2d(dx, w, h, 4.0, 3.0, name=f"ensemble-{i} dx={dx} {w:.2f}x{h:.2f} m"))
+    return out
+
+
 def load_dualsphysics_csv(fluid_csv: str, boundary_csv: str, cols: Sequence[str] = ("Points:0", "Points:2")):
     """Reads the reference's CSV fixtures (src/auxillary.jl:3-11): points, Rhop, ptype."""
     pts: List[List[float]] = [[] for _ in cols]

@@ -270,3 +270,34 @@ def test_full_size_f32split_vs_f64_1000_steps(G, S):
     assert np.quantile(err, 0.9999) <= np.quantile(chaos, 0.9999)
     assert err.max() <= floor
     assert t32 == pytest.approx(t64, rel=1e-6)
+
+
+@pytest.mark.parametrize("case", ["dam2d_allphys", "dam3d_allphys"])
+def test_graph_replay_equals_eager_launches(G, monkeypatch, case):
+    """Calls of several steps replay a captured step graph (kept across calls
+    while nothing it bakes in changes); GCLSPH_NO_GRAPH=1 launches every kernel
+    eagerly.  Several calls with eager work in between (a single-step call,
+    parameter edits, getters) give bit-identical states."""
+    sc, steps, ts = golden_cases()[case]
+
+    def run(no_graph):
+        if no_graph:
+            monkeypatch.setenv("GCLSPH_NO_GRAPH", "1")
+        else:
+            monkeypatch.delenv("GCLSPH_NO_GRAPH", raising=False)
+        cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, v=sc.v, precision="f32split")
+        out = []
+        for n in (4, 1, 3, 3):
+            out.append(G.stepsolve_(prob, n, timestepping=ts))
+            out.append(np.array(G.get_points(prob)))
+            out.append(G.get_density(prob))
+        prob.alpha = 0.02  # a changed constant re-captures the step
+        out.append(G.stepsolve_(prob, 3, timestepping=ts))
+        out += [np.asarray(gpu_fields(G, prob)[f]) for f in FIELDS]
+        return out
+
+    ref = run(True)
+    got = run(False)
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
