@@ -421,6 +421,14 @@ struct sph_s {
     C.f_st_c = float(C.st_c);
     C.f_st_01h = float(C.st_01h);
     C.f_xsph_c = float(C.xsph_c);
+    C.f_dpc_l0inv = float(C.dpc_l0inv);
+    C.f_dpc_eta2 = float(C.dpc_eta2);
+    C.f_dpc_whinv = float(C.dpc_whinv);
+    C.f_dpc_lim = float(C.dpc_lim);
+    C.f_dpc_wnorm = float(C.dpc_wnorm);
+    C.f_dpc_lambda = float(C.dpc_lambda);
+    C.f_dpc_pmin = float(C.dpc_pmin);
+    C.f_dpc_pmax = float(C.dpc_pmax);
     return C;
   }
 

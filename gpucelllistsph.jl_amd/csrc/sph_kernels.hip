@@ -1125,28 +1125,52 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
   const double* XM[3];
   membership_ptrs<D>(B, F, p, rebuilt, XM);
   const int32_t pta = PT[a];
-  const double rhoa = std::is_same<R, double>::value ? double(RHO[a]) : C.rho0 + double(RHO[a]);
-  const double Pa = double(B.PN[a]);
-  double sum[3] = {0.0, 0.0, 0.0};
   const int nc = B.NC[a];
-  for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
-      const int32_t ptb = PT[b];
-      if (!(pta > 0 && ptb > 0)) return;
+  double xa[3];
+  R va[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    xa[d] = X[d][a];
+    va[d] = V[d][a];
+  }
+  struct Nb {
+    double x[3];
+    R v[3], rho, P;
+    int32_t pt;
+  };
+  auto fetch = [&](const int32_t b) {
+    const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
+    Nb n;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      n.x[d] = ldo(X[d], o8);
+      n.v[d] = ldo(V[d], oR);
+    }
+    n.rho = ldo(RHO, oR);
+    n.P = ldo<R>(B.PN, oR);
+    n.pt = ldo(PT, o4);
+    return n;
+  };
+  const R Pa = B.PN[a];
+  if constexpr (std::is_same<R, double>::value) {
+    const double rhoa = RHO[a];
+    double sum[3] = {0.0, 0.0, 0.0};
+    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+      if (!(pta > 0 && n.pt > 0)) return;
       double dx[3], dv[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        dx[d] = X[d][a] - X[d][b];
-        dv[d] = double(V[d][a]) - double(V[d][b]);
+        dx[d] = xa[d] - n.x[d];
+        dv[d] = va[d] - n.v[d];
       }
-      const double rhob = std::is_same<R, double>::value ? double(RHO[b]) : C.rho0 + double(RHO[b]);
-      const double Pb = double(B.PN[b]);
+      const double rhob = n.rho;
       const double r2 = dotd<D>(dx, dx);
       const double r = sqrt(r2);
       const double u = r * C.dpc_l0inv;
       const double w = W_d(u, C.dpc_wnorm);
       const double chi = sqrt(w * C.dpc_whinv);
       const double kk = u < C.dpc_lim ? 1.0 : chi;
-      double lo = C.dpc_lambda * fabs(Pa + Pb), cap = C.dpc_lambda * C.dpc_pmax;
+      double lo = C.dpc_lambda * fabs(Pa + n.P), cap = C.dpc_lambda * C.dpc_pmax;
       double pt = lo < cap ? lo : cap;
       pt = pt > C.dpc_pmin ? pt : C.dpc_pmin;
       const double Pb_dpc = chi * pt;
@@ -1160,10 +1184,48 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
 #pragma unroll
         for (int d = 0; d < D; ++d) sum[d] += tvar * dx[d];
       }
-  });
+    });
 #pragma unroll
-  for (int d = 0; d < D; ++d) B.DV[d][a] = R(sum[d]);
-  if (F.want_out) B.PBUF[a] = R(Pa);
+    for (int d = 0; d < D; ++d) B.DV[d][a] = sum[d];
+  } else {
+#pragma clang fp contract(fast)
+    const float rhoa = C.f_rho0 + RHO[a];
+    const float fdt = float(dt);
+    Sum<float> sum[3];
+    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+      if (!(pta > 0 && n.pt > 0)) return;
+      float dx[3], dv[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        dx[d] = float(xa[d] - n.x[d]);
+        dv[d] = va[d] - n.v[d];
+      }
+      const float rhob = C.f_rho0 + n.rho;
+      const float r2 = dotf<D>(dx, dx);
+      const float u = __builtin_amdgcn_sqrtf(r2) * C.f_dpc_l0inv;
+      const float w = W_f(u, C.f_dpc_wnorm);
+      const float chi = __builtin_amdgcn_sqrtf(w * C.f_dpc_whinv);
+      const float kk = u < C.f_dpc_lim ? 1.0f : chi;
+      const float lo = C.f_dpc_lambda * fabsf(Pa + n.P), cap = C.f_dpc_lambda * C.f_dpc_pmax;
+      float pt = lo < cap ? lo : cap;
+      pt = pt > C.f_dpc_pmin ? pt : C.f_dpc_pmin;
+      const float Pb_dpc = chi * pt;
+      const float vr = dotf<D>(dv, dx);
+      const float den = __builtin_amdgcn_rcpf(r2 + C.f_dpc_eta2);
+      if (vr < 0.0f) {
+        const float vrdr = vr * den;
+#pragma unroll
+        for (int d = 0; d < D; ++d) sum[d].add(kk * (-vrdr * dx[d]));
+      } else {
+        const float tvar = 2.0f * fdt * Pb_dpc * __builtin_amdgcn_rcpf(rhoa + rhob) * den;
+#pragma unroll
+        for (int d = 0; d < D; ++d) sum[d].add(tvar * dx[d]);
+      }
+    });
+#pragma unroll
+    for (int d = 0; d < D; ++d) B.DV[d][a] = sum[d].get();
+  }
+  if (F.want_out) B.PBUF[a] = Pa;
 }
 
 // update_dpcreg! (src/gpukernels_sph.jl:1617-1632)
@@ -1202,23 +1264,68 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
   const R* RHO = B.RHO[q];
   const double* XM[3];
   membership_ptrs<D>(B, F, p, rebuilt, XM);
-  const double rhoa = std::is_same<R, double>::value ? double(RHO[a]) : C.rho0 + double(RHO[a]);
-  double sum[3] = {0.0, 0.0, 0.0};
   const int nc = B.NC[a];
-  for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
-      if (PT[b] <= 1) return;
+  double xa[3];
+  R va[3];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    xa[d] = XN[d][a];
+    va[d] = V[d][a];
+  }
+  struct Nb {
+    double x[3];
+    R v[3], rho;
+    int32_t pt;
+  };
+  auto fetch = [&](const int32_t b) {
+    const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
+    Nb n;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      n.x[d] = ldo(XN[d], o8);
+      n.v[d] = ldo(V[d], oR);
+    }
+    n.rho = ldo(RHO, oR);
+    n.pt = ldo(PT, o4);
+    return n;
+  };
+  double out[3];
+  if constexpr (std::is_same<R, double>::value) {
+    const double rhoa = RHO[a];
+    double sum[3] = {0.0, 0.0, 0.0};
+    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+      if (n.pt <= 1) return;
       double dx[3];
 #pragma unroll
-      for (int d = 0; d < D; ++d) dx[d] = XN[d][a] - XN[d][b];
+      for (int d = 0; d < D; ++d) dx[d] = xa[d] - n.x[d];
       const double r = sqrt(dotd<D>(dx, dx));
       const double w = W_d(r * C.Hinv, C.wnorm);
-      const double rhob = std::is_same<R, double>::value ? double(RHO[b]) : C.rho0 + double(RHO[b]);
-      const double xs = C.xsph_c * w / (rhoa + rhob);
+      const double xs = C.xsph_c * w / (rhoa + n.rho);
 #pragma unroll
-      for (int d = 0; d < D; ++d) sum[d] += (double(V[d][b]) - double(V[d][a])) * xs;
-  });
+      for (int d = 0; d < D; ++d) sum[d] += (n.v[d] - va[d]) * xs;
+    });
 #pragma unroll
-  for (int d = 0; d < D; ++d) B.X[q][d][a] = B.X[q][d][a] + sum[d] * dt;
+    for (int d = 0; d < D; ++d) out[d] = sum[d];
+  } else {
+#pragma clang fp contract(fast)
+    const float rhoa = C.f_rho0 + RHO[a];
+    Sum<float> sum[3];
+    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+      if (n.pt <= 1) return;
+      float dx[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) dx[d] = float(xa[d] - n.x[d]);
+      const float r = __builtin_amdgcn_sqrtf(dotf<D>(dx, dx));
+      const float w = W_f(r * C.f_Hinv, C.f_wnorm);
+      const float xs = C.f_xsph_c * w * __builtin_amdgcn_rcpf(rhoa + (C.f_rho0 + n.rho));
+#pragma unroll
+      for (int d = 0; d < D; ++d) sum[d].add((n.v[d] - va[d]) * xs);
+    });
+#pragma unroll
+    for (int d = 0; d < D; ++d) out[d] = double(sum[d].get());
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) B.X[q][d][a] = B.X[q][d][a] + out[d] * dt;
 }
 
 // cspmcorr! (src/gpukernels_sph.jl:1693-1719): ρ = Σ m0 W / Σ m0 W / ρ_b, W at xⁿ.
@@ -1239,18 +1346,52 @@ __global__ __launch_bounds__(256) void k_cspm_sum(SS ss) {
   double s1 = 0.0, s2 = 0.0;
   if (pta > 0) {
     const int nc = B.NC[a];
-    for_each_partner<D>(B, a, nc, G, XM, C.dist2, [&](const int32_t b) {
-        if (PT[b] <= 0) return;
+    double xa[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) xa[d] = XN[d][a];
+    struct Nb {
+      double x[3];
+      R rho;
+      int32_t pt;
+    };
+    auto fetch = [&](const int32_t b) {
+      const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
+      Nb n;
+#pragma unroll
+      for (int d = 0; d < D; ++d) n.x[d] = ldo(XN[d], o8);
+      n.rho = ldo(RHO, oR);
+      n.pt = ldo(PT, o4);
+      return n;
+    };
+    if constexpr (std::is_same<R, double>::value) {
+      for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+        if (n.pt <= 0) return;
         double dx[3];
 #pragma unroll
-        for (int d = 0; d < D; ++d) dx[d] = XN[d][a] - XN[d][b];
+        for (int d = 0; d < D; ++d) dx[d] = xa[d] - n.x[d];
         const double w = W_d(sqrt(dotd<D>(dx, dx)) * C.Hinv, C.wnorm);
-        const double rhob = std::is_same<R, double>::value ? double(RHO[b]) : C.rho0 + double(RHO[b]);
         s1 += C.m0 * w;
-        s2 += w * C.m0 / rhob;
-    });
+        s2 += w * C.m0 / n.rho;
+      });
+    } else {
+      // on δρ: with e = δρ_b/ρ0, ρ' - ρ0 = ρ0 Σ m0W e/(1+e) / Σ m0W/(1+e), which
+      // is s1/s2 - ρ0 without the cancellation against ρ0
+#pragma clang fp contract(fast)
+      for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+        if (n.pt <= 0) return;
+        float dx[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) dx[d] = float(xa[d] - n.x[d]);
+        const float w = W_f(__builtin_amdgcn_sqrtf(dotf<D>(dx, dx)) * C.f_Hinv, C.f_wnorm);
+        const float e = n.rho * (1.0f / C.f_rho0);
+        const float mw = C.f_m0 * w * __builtin_amdgcn_rcpf(1.0f + e);
+        s1 += double(mw * e);
+        s2 += double(mw);
+      });
+      s1 *= C.rho0;
+    }
   }
-  B.TMP[a] = s1 / s2;
+  B.TMP[a] = s1 / s2;  // ρ (FP64 path) or ρ - ρ0 (FP32-split); NaN without partners
 }
 
 template <typename R, typename SS>
@@ -1267,7 +1408,7 @@ __global__ void k_cspm_apply(SS ss) {
     B.RHO[q][a] = nr;
     B.PN[a] = eos_d(nr, pt, C);
   } else {
-    float d = float(nr - C.rho0);
+    const float d = float(nr);
     B.RHO[q][a] = d;
     B.PN[a] = eos_f(d, pt, C);
   }
