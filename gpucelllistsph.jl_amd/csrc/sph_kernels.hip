@@ -315,25 +315,111 @@ __device__ __forceinline__ void row_range(const CellGeom& g, const int32_t* __re
   e = in ? start[row + xh + 1] : 0;
 }
 
+// window_bits over candidate positions staged in LDS (2D: (x, y) pairs): lp
+// points at candidate lo.  Same arithmetic, so the same bits.
+__device__ __forceinline__ uint32_t window_bits_lds(const double2* lp, double xa0, double xa1, int64_t a,
+                                                    int32_t lo, int32_t hi, double dist2) {
+#pragma clang fp contract(off)
+  const int32_t len = hi - lo;
+  uint32_t mk = 0;
+  for (int32_t j = 0; j < len; j += 4) {
+    double2 c[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = lp[j + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double d0 = xa0 - c[u].x;
+      const double d1 = xa1 - c[u].y;
+      const double r2 = d0 * d0 + d1 * d1;
+      mk |= uint32_t(r2 < dist2) << (j + u);
+    }
+  }
+  if (len < 32) mk &= (1u << len) - 1u;
+  const int64_t self = a - lo;
+  if (self >= 0 && self < len) mk &= ~(1u << self);
+  return mk;
+}
+
+// Candidates staged in LDS per block (2D): 2048 positions, 32 KiB.
+constexpr int kNmaskLds = 2048;
+
 // Neighbour-list membership of the last rebuild, stored per particle as
 // (mask, base) entries: one 32-candidate window [base, base+32) of one stencil
 // row with its member bits in the low word.  Only windows with members are
 // stored, in row order.  Entry e of particle a is NE[e * n + a]; NC[a] is the
 // entry count, or -1 when the particle needs more than necap entries (its
 // partners are then found by scanning the stencil).
+//
+// 2D: the block's 256 consecutive particles lie in one contiguous range of
+// cells, so the union of their stencil rows is, per row offset, one contiguous
+// range of sorted particles.  The block stages those positions in LDS once
+// (coalesced) and every lane tests its candidates there, instead of each lane
+// fetching each of its ~100 candidates through the vector L1.  A block whose
+// union exceeds the LDS budget reads the candidates from global memory.
 template <int D, typename R, typename SS>
 __global__ __launch_bounds__(256) void k_nmask(SS ss) {
   GCLSPH_SCENE(ss);
   const StepCtl* ctl = B.ctl;
   if (!ctl->rebuilt) return;
-  if (a >= B.n) return;
+  const int64_t a0 = int64_t(m.y) * kBlock;
+  if (a0 >= B.n) return;  // whole block
+  const bool active = a < B.n;
   const double dist2 = C.dist2;
   const int p = ctl->parity;  // the rebuild positions are the current ones
   const double* XM[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
-  const double xa0 = XM[0][a], xa1 = XM[1][a], xa2 = D == 3 ? XM[2][a] : 0.0;
-  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
   const int cap = B.necap;
   uint64_t* out = B.NE + a;
+  if constexpr (D == 2) {
+    __shared__ double2 lds[kNmaskLds + 4];  // +4: window reads run up to 3 past a row's end
+    __shared__ int32_t ub[3], uo[4];
+    if (threadIdx.x == 0) {
+      const int64_t a1 = (a0 + kBlock < B.n ? a0 + kBlock : B.n) - 1;
+      const int64_t c0 = B.skey[a0], c1 = B.skey[a1];
+      const int64_t gx = G.grid[0], nc = G.ncells();
+      int32_t o = 0;
+      for (int k = 0; k < 3; ++k) {
+        int64_t lo = c0 + (k - 1) * gx - 1, hi = c1 + (k - 1) * gx + 1;
+        lo = lo < 0 ? 0 : (lo > nc - 1 ? nc - 1 : lo);
+        hi = hi < 0 ? 0 : (hi > nc - 1 ? nc - 1 : hi);
+        const int32_t b = B.start[lo], e = B.start[hi + 1];
+        ub[k] = b;
+        uo[k] = o;
+        o += e > b ? e - b : 0;
+      }
+      uo[3] = o;
+    }
+    __syncthreads();
+    const int32_t total = uo[3];
+    if (total <= kNmaskLds) {  // uniform over the block
+      for (int32_t i = threadIdx.x; i < total; i += kBlock) {
+        const int k = (i >= uo[1]) + (i >= uo[2]);
+        const int32_t b = ub[k] + (i - uo[k]);
+        lds[i] = make_double2(XM[0][b], XM[1][b]);
+      }
+      __syncthreads();
+      if (!active) return;
+      const double xa0 = XM[0][a], xa1 = XM[1][a];
+      const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
+      int cnt = 0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double2* lrow = lds + (uo[k] - ub[k]);  // candidate b at lrow[b]
+        for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
+          const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
+          const uint32_t mk = window_bits_lds(lrow + w0, xa0, xa1, a, w0, hi, dist2);
+          if (mk != 0) {
+            if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
+            ++cnt;
+          }
+        }
+      }
+      B.NC[a] = cnt <= cap ? cnt : -1;
+      return;
+    }
+  }
+  if (!active) return;
+  const double xa0 = XM[0][a], xa1 = XM[1][a], xa2 = D == 3 ? XM[2][a] : 0.0;
+  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
   int cnt = 0;
 #pragma unroll
   for (int k = 0; k < Rows<D>::N; ++k) {
@@ -1152,11 +1238,17 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
     return n;
   };
   const R Pa = B.PN[a];
+  if (F.want_out) B.PBUF[a] = Pa;
+  if (pta <= 0) {  // every DPC pair needs ptype > 0 on both sides
+#pragma unroll
+    for (int d = 0; d < D; ++d) B.DV[d][a] = R(0);
+    return;
+  }
   if constexpr (std::is_same<R, double>::value) {
     const double rhoa = RHO[a];
     double sum[3] = {0.0, 0.0, 0.0};
     for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
-      if (!(pta > 0 && n.pt > 0)) return;
+      if (n.pt <= 0) return;
       double dx[3], dv[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -1193,7 +1285,7 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
     const float fdt = float(dt);
     Sum<float> sum[3];
     for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
-      if (!(pta > 0 && n.pt > 0)) return;
+      if (n.pt <= 0) return;
       float dx[3], dv[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -1225,7 +1317,6 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
 #pragma unroll
     for (int d = 0; d < D; ++d) B.DV[d][a] = sum[d].get();
   }
-  if (F.want_out) B.PBUF[a] = Pa;
 }
 
 // update_dpcreg! (src/gpukernels_sph.jl:1617-1632)
