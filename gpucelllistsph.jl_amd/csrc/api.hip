@@ -7,6 +7,7 @@
 // steps with no host synchronisation until the batch ends.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -721,12 +722,40 @@ void run_steps(sph_s* const* P, unsigned count, LaunchSet& L, hipStream_t st, in
   sph_s* timer = (count == 1 && P[0]->timing) ? P[0] : nullptr;
   bool any_cspmn = false;
   for (unsigned i = 0; i < count; ++i) any_cspmn |= flags[i].cspmn > 0;
+  // Pass variants (same results, different code generation): forced by
+  // GCLSPH_PASS_VARIANT=<pass1>,<pass2>, else the ones this launch set chose by
+  // timing, else the fastest measured on the dam-break scenes per dimension
+  // and precision until it has been tuned.
+  int pv[2];
+  if (D == 2) {
+    pv[0] = pv[1] = std::is_same<R, float>::value ? 3 : 2;
+  } else {
+    pv[0] = std::is_same<R, float>::value ? 0 : 1;
+    pv[1] = 0;
+  }
+  bool forced = false;
+  if (const char* env = std::getenv("GCLSPH_PASS_VARIANT")) {
+    int v1 = -1, v2 = -1;
+    if (std::sscanf(env, "%d,%d", &v1, &v2) == 2 && v1 >= 0 && v1 < kPassVariants && v2 >= 0 &&
+        v2 < kPassVariants) {
+      pv[0] = v1;
+      pv[1] = v2;
+      forced = true;
+    }
+  }
+  if (!forced && L.pass_variant[0] >= 0) {
+    pv[0] = L.pass_variant[0];
+    pv[1] = L.pass_variant[1];
+  }
   // one step's launches; with the corrections of every problem that may need
   // them (each kernel returns at once for a problem that does not)
-  auto launch_step = [&](bool cspm) {
+  auto launch_step = [&](bool cspm, int v1, int v2, hipEvent_t* ev) {
     launch_rebuild<D, R>(L, st);
-    launch_pass1<D, R>(L, st);
-    launch_pass2<D, R>(L, st);
+    if (ev) GCL_HIP(hipEventRecord(ev[0], st));
+    launch_pass1<D, R>(L, st, v1);
+    if (ev) GCL_HIP(hipEventRecord(ev[1], st));
+    launch_pass2<D, R>(L, st, v2);
+    if (ev) GCL_HIP(hipEventRecord(ev[2], st));
     if (any_dpc) launch_dpc<D, R>(L, st);
     if (any_xsph) launch_xsph<D, R>(L, st);
     if (cspm) launch_cspm<D, R>(L, st);
@@ -734,14 +763,49 @@ void run_steps(sph_s* const* P, unsigned count, LaunchSet& L, hipStream_t st, in
     launch_finalize<D, R>(L, st);
     GCL_HIP(hipGetLastError());
   };
+  auto cspm_at = [&](int64_t it) {
+    bool any = false;
+    for (unsigned i = 0; i < count; ++i) any |= flags[i].cspmn > 0 && it % flags[i].cspmn == 0;
+    return any;
+  };
+  int64_t done = 0;
+  const char* no_tune = std::getenv("GCLSPH_AUTOTUNE");
+  if (!forced && L.pass_variant[0] < 0 && !timer && nsteps > kPassVariants && !(no_tune && no_tune[0] == '0')) {
+    // Autotune: the first kPassVariants steps of the call run one variant each
+    // (live steps: every variant computes the same values), timing both
+    // passes; the fastest variant of each pass is kept for this launch set.
+    std::vector<hipEvent_t> ev(3 * kPassVariants, nullptr);
+    for (auto& e : ev) GCL_HIP(hipEventCreate(&e));
+    for (int v = 0; v < kPassVariants; ++v) launch_step(cspm_at(v + 1), v, v, &ev[3 * size_t(v)]);
+    GCL_HIP(hipStreamSynchronize(st));
+    float best[2] = {0, 0};
+    for (int v = 0; v < kPassVariants; ++v) {
+      float t1 = 0, t2 = 0;
+      GCL_HIP(hipEventElapsedTime(&t1, ev[3 * size_t(v)], ev[3 * size_t(v) + 1]));
+      GCL_HIP(hipEventElapsedTime(&t2, ev[3 * size_t(v) + 1], ev[3 * size_t(v) + 2]));
+      if (v == 0 || t1 < best[0]) {
+        best[0] = t1;
+        pv[0] = v;
+      }
+      if (v == 0 || t2 < best[1]) {
+        best[1] = t2;
+        pv[1] = v;
+      }
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    L.pass_variant[0] = pv[0];
+    L.pass_variant[1] = pv[1];
+    done = kPassVariants;
+  }
   const char* no_graph = std::getenv("GCLSPH_NO_GRAPH");
-  const bool graph = !timer && nsteps >= 2 && !(no_graph && no_graph[0] == '1');
+  const bool graph = !timer && nsteps - done >= 2 && !(no_graph && no_graph[0] == '1');
   if (graph) {
-    // The step as a captured graph, replayed nsteps times: every per-step
-    // decision lives in the control blocks, so one capture serves all steps.
-    // Kept for the next call while the launches and one-problem descriptor
-    // (baked into the graph) are unchanged.
-    std::vector<unsigned char> key = {uint8_t(any_dpc), uint8_t(any_xsph), uint8_t(any_cspmn), uint8_t(any_dtv)};
+    // The step as a captured graph, replayed for the remaining steps: every
+    // per-step decision lives in the control blocks, so one capture serves all
+    // steps.  Kept for the next call while the launches, pass variants and
+    // one-problem descriptor (baked into the graph) are unchanged.
+    std::vector<unsigned char> key = {uint8_t(any_dpc), uint8_t(any_xsph), uint8_t(any_cspmn), uint8_t(any_dtv),
+                                      uint8_t(pv[0]), uint8_t(pv[1])};
     if (L.one) {
       const unsigned char* d = static_cast<const unsigned char*>(L.one);
       key.insert(key.end(), d, d + sizeof(SceneDev<R>));
@@ -751,24 +815,23 @@ void run_steps(sph_s* const* P, unsigned count, LaunchSet& L, hipStream_t st, in
       L.step_graph = nullptr;
       hipGraph_t g = nullptr;
       GCL_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-      launch_step(any_cspmn);
+      launch_step(any_cspmn, pv[0], pv[1], nullptr);
       GCL_HIP(hipStreamEndCapture(st, &g));
       const hipError_t e = hipGraphInstantiate(&L.step_graph, g, nullptr, nullptr, 0);
       (void)hipGraphDestroy(g);
       GCL_HIP(e);
       L.graph_key = std::move(key);
     }
-    for (int64_t it = 1; it <= nsteps; ++it) GCL_HIP(hipGraphLaunch(L.step_graph, st));
+    for (int64_t it = done + 1; it <= nsteps; ++it) GCL_HIP(hipGraphLaunch(L.step_graph, st));
   } else {
     auto mark_begin = [&]() { return timer ? timer->mark_begin(st) : hipEvent_t(nullptr); };
     auto mark_end = [&](int stage, hipEvent_t e) {
       if (timer) timer->mark_end(stage, e, st);
     };
-    for (int64_t it = 1; it <= nsteps; ++it) {
-      bool any_cspm = false;
-      for (unsigned i = 0; i < count; ++i) any_cspm |= flags[i].cspmn > 0 && it % flags[i].cspmn == 0;
+    for (int64_t it = done + 1; it <= nsteps; ++it) {
+      const bool any_cspm = cspm_at(it);
       if (!timer) {
-        launch_step(any_cspm);
+        launch_step(any_cspm, pv[0], pv[1], nullptr);
         continue;
       }
       hipEvent_t e0 = mark_begin();
@@ -776,11 +839,11 @@ void run_steps(sph_s* const* P, unsigned count, LaunchSet& L, hipStream_t st, in
       GCL_HIP(hipGetLastError());
       mark_end(0, e0);
       e0 = mark_begin();
-      launch_pass1<D, R>(L, st);
+      launch_pass1<D, R>(L, st, pv[0]);
       GCL_HIP(hipGetLastError());
       mark_end(1, e0);
       e0 = mark_begin();
-      launch_pass2<D, R>(L, st);
+      launch_pass2<D, R>(L, st, pv[1]);
       GCL_HIP(hipGetLastError());
       mark_end(2, e0);
       if (any_dpc || any_xsph || any_cspm || any_dtv) {

@@ -125,6 +125,8 @@ struct LaunchSet {
   int2* cmap = nullptr;
   unsigned pblocks = 0, ctiles = 0, count = 0;
   size_t scene_bytes = 0;
+  // pass variants chosen by timing (-1: not yet)
+  int pass_variant[2] = {-1, -1};
   // the captured step of the last call and what it was captured for
   hipGraphExec_t step_graph = nullptr;
   std::vector<unsigned char> graph_key;
@@ -137,10 +139,12 @@ struct LaunchSet {
 // binning, scan, scatter, rank, reorder of the state, commit, membership store.
 template <int D, typename R>
 void launch_rebuild(const LaunchSet& L, hipStream_t s);
+// Pair passes in one of kPassVariants code-generation variants (same results).
+constexpr int kPassVariants = 4;
 template <int D, typename R>
-void launch_pass1(const LaunchSet& L, hipStream_t s);
+void launch_pass1(const LaunchSet& L, hipStream_t s, int variant);
 template <int D, typename R>
-void launch_pass2(const LaunchSet& L, hipStream_t s);
+void launch_pass2(const LaunchSet& L, hipStream_t s, int variant);
 template <int D, typename R>
 void launch_dpc(const LaunchSet& L, hipStream_t s);
 template <int D, typename R>

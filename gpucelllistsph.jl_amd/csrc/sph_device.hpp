@@ -1,0 +1,433 @@
+// Device helpers of the SPH step shared by the pass kernels (sph_pass.hip)
+// and the other step kernels (sph_kernels.hip): SPH kernel functions, EOS,
+// accumulators, scene sources, the neighbour-membership walk and reductions.
+#pragma once
+
+#include <type_traits>
+
+#include "blockscan.hpp"
+#include "sph.hpp"
+
+namespace gclsph {
+
+// ---------------------------------------------------------------- math helpers
+__device__ __forceinline__ double powi_d(double x, int p) {
+#pragma clang fp contract(off)
+  // binary powering, same operation sequence as the oracle's powi
+  double r = 1.0, b = x;
+  bool first = true;
+  while (p > 0) {
+    if (p & 1) {
+      r = first ? b : r * b;
+      first = false;
+    }
+    p >>= 1;
+    if (p) b = b * b;
+  }
+  return r;
+}
+
+// powfancy7th (src/gpukernels_sph.jl:499-515) for γ == 7; x^(1/γ) otherwise.
+__device__ __forceinline__ double pf7(double x, int gamma, double ginv) {
+#pragma clang fp contract(off)
+  if (gamma == 7) {
+    unsigned long long bits = (unsigned long long)__double_as_longlong(fabs(x));
+    unsigned long long tb = 0x36cd000000000000ull + bits / 7ull;
+    double t = copysign(__longlong_as_double((long long)tb), x);
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+      double t2 = t * t;
+      double t3 = t2 * t;
+      double t4 = t2 * t2;
+      double xot4 = x / t4;
+      t = t - t * (t3 - xot4) / (4.0 * t3 + 3.0 * xot4);
+    }
+    return t;
+  }
+  return pow(x, ginv);
+}
+
+// WendlandC2 (SPHKernels.jl): W = ((1+4u)(1-u)^4) σ H⁻ᴰ, dW = -20u(1-u)^3 σ H⁻⁽ᴰ⁺¹⁾.
+__device__ __forceinline__ double W_d(double u, double wnorm) {
+#pragma clang fp contract(off)
+  if (u < 1.0) {
+    double um1 = 1.0 - u;
+    um1 = um1 * um1;
+    um1 = um1 * um1;
+    return ((1.0 + 4.0 * u) * um1) * wnorm;
+  }
+  return 0.0;
+}
+__device__ __forceinline__ double dW_d(double u, double dwnorm) {
+#pragma clang fp contract(off)
+  if (u < 1.0) {
+    double um1 = 1.0 - u;
+    double um13 = um1 * um1 * um1;
+    return -20.0 * u * um13 * dwnorm;
+  }
+  return 0.0;
+}
+__device__ __forceinline__ float W_f(float u, float wnorm) {
+  if (u < 1.0f) {
+    float um1 = 1.0f - u;
+    um1 = um1 * um1;
+    um1 = um1 * um1;
+    return ((1.0f + 4.0f * u) * um1) * wnorm;
+  }
+  return 0.0f;
+}
+__device__ __forceinline__ float dW_f(float u, float dwnorm) {
+  if (u < 1.0f) {
+    float um1 = 1.0f - u;
+    return -20.0f * u * (um1 * um1 * um1) * dwnorm;
+  }
+  return 0.0f;
+}
+
+// Tait EOS (src/gpukernels_sph.jl:691-697): 0 if ptype <= 1 and ρ < ρ0.
+__device__ __forceinline__ double eos_d(double rho, int32_t pt, const SphConst& C) {
+#pragma clang fp contract(off)
+  if (pt <= 1 && rho < C.rho0) return 0.0;
+  return C.P0 * (powi_d(rho / C.rho0, C.gamma) - 1);
+}
+// FP32-split EOS on δρ: P0((1+e)^7 - 1) = P0 e (7 + 21e + 35e² + 35e³ + 21e⁴ + 7e⁵ + e⁶).
+__device__ __forceinline__ float eos_f(float drho, int32_t pt, const SphConst& C) {
+  if (pt <= 1 && drho < 0.0f) return 0.0f;
+  float e = drho / C.f_rho0;
+  float poly = 7.0f + e * (21.0f + e * (35.0f + e * (35.0f + e * (21.0f + e * (7.0f + e)))));
+  return C.f_P0 * (e * poly);
+}
+// ρ0((1+z)^(1/7) - 1) without cancellation (FP32-split form of
+// ρ0*powfancy7th(1+z) - ρ0): binomial series, exact function for |z| > 1e-2.
+__device__ __forceinline__ float hydro_drho_f(float z, float rho0) {
+  if (fabsf(z) < 1e-2f) {
+    float t = z * (1.0f / 7.0f + z * (-3.0f / 49.0f + z * (13.0f / 343.0f + z * (-65.0f / 2401.0f))));
+    return rho0 * t;
+  }
+  return rho0 * expm1f(log1pf(z) * (1.0f / 7.0f));
+}
+
+template <typename R>
+struct Sum {  // plain sum
+  R s = R(0);
+  __device__ __forceinline__ void add(R v) { s += v; }
+  __device__ __forceinline__ R get() const { return s; }
+};
+template <>
+struct Sum<float> {  // FP32 pair terms accumulated in FP64 (2 VALU per term, exact for ~30 terms)
+  double s = 0.0;
+  __device__ __forceinline__ void add(float v) { s += double(v); }
+  __device__ __forceinline__ float get() const { return float(s); }
+};
+
+template <int D>
+__device__ __forceinline__ bool member(const double* const* XM, int64_t a, int64_t b, double dist2) {
+#pragma clang fp contract(off)
+  double d0 = XM[0][a] - XM[0][b];
+  double d1 = XM[1][a] - XM[1][b];
+  double s = d0 * d0 + d1 * d1;
+  if (D == 3) {
+    double d2 = XM[2][a] - XM[2][b];
+    s = s + d2 * d2;
+  }
+  return s < dist2;
+}
+
+template <int D>
+__device__ __forceinline__ double dotd(const double* a, const double* b) {
+#pragma clang fp contract(off)
+  double s = a[0] * b[0] + a[1] * b[1];
+  if (D == 3) s = s + a[2] * b[2];
+  return s;
+}
+template <int D>
+__device__ __forceinline__ float dotf(const float* a, const float* b) {
+  float s = a[0] * b[0] + a[1] * b[1];
+  if (D == 3) s = s + a[2] * b[2];
+  return s;
+}
+
+// Positions of the pair-list membership test: the current positions when this
+// step rebuilt (or when no later step can skip a rebuild), else the stored
+// rebuild positions.
+template <int D, typename R>
+__device__ __forceinline__ void membership_ptrs(const SphBufs<R>& B, const StepFlags& F, int p, int rebuilt,
+                                                const double** XM) {
+  bool cur = rebuilt || !F.need_xrb;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) XM[k] = cur ? B.X[p][k < D ? k : 1] : B.XRB[k < D ? k : 1];
+}
+
+// Logical number of this block with an XCD-aware order: workgroups are dealt
+// to the 8 XCDs round robin (block b runs on XCD b % 8), and renumbering so
+// that each XCD walks one contiguous range of logical blocks (cell-sorted
+// particles) keeps the neighbourhoods shared by consecutive blocks, and by the
+// stencil rows above and below, inside one XCD's L2.
+__device__ __forceinline__ unsigned xcd_block() {
+  const unsigned G = gridDim.x, b = blockIdx.x;
+  const unsigned q = G / 8, r = G % 8, xcd = b % 8, i = b / 8;
+  return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
+}
+
+// Where a kernel finds its problem.  One problem: the descriptor is a kernel
+// argument (constant memory the compiler may reload at will, which keeps
+// register pressure as low as plain arguments).  Several problems: an array of
+// descriptors in device memory and maps from blocks to (problem, block of the
+// problem) or (problem, cell tile).
+template <typename R>
+struct OneScene {
+  SceneDev<R> d;
+};
+template <typename R>
+struct ManyScenes {
+  const SceneDev<R>* __restrict__ S;
+  const int2* __restrict__ pmap;
+  const int2* __restrict__ cmap;
+};
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& particle_scene(const OneScene<R>& s, int2& m) {
+  m = make_int2(0, int(xcd_block()));
+  return s.d;
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& particle_scene(const ManyScenes<R>& s, int2& m) {
+  m = s.pmap[xcd_block()];
+  return s.S[m.x];
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& tile_scene(const OneScene<R>& s, int2& m) {
+  m = make_int2(0, int(blockIdx.x));
+  return s.d;
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& tile_scene(const ManyScenes<R>& s, int2& m) {
+  m = s.cmap[blockIdx.x];
+  return s.S[m.x];
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& scene_at(const OneScene<R>& s, unsigned) {
+  return s.d;
+}
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& scene_at(const ManyScenes<R>& s, unsigned i) {
+  return s.S[i];
+}
+
+// Per-step switches of a problem, from its per-call switches and control block.
+template <typename R>
+__device__ __forceinline__ StepFlags step_flags(const SceneDev<R>& sd) {
+  StepFlags F = sd.F;
+  const StepCtl* c = sd.B.ctl;
+  const int it = c->iter;
+  F.synced_prev = c->synced;
+  F.cspm_fire = F.cspmn > 0 && it % F.cspmn == 0;
+  F.corr_on = F.dpc_on || F.xsph_on || F.cspm_fire;
+  F.want_out = it == c->nsteps;
+  return F;
+}
+
+// Binds the problem of this block: B, C, G, F and the particle index a.
+#define GCLSPH_SCENE(ss)                                       \
+  int2 m;                                                      \
+  const SceneDev<R>& sd = particle_scene<R>(ss, m);            \
+  const SphBufs<R>& B = sd.B;                                  \
+  const SphConst& C = sd.C;                                    \
+  const CellGeom& G = sd.G;                                    \
+  const StepFlags F = step_flags(sd);                          \
+  const int64_t a = int64_t(m.y) * kBlock + threadIdx.x;       \
+  (void)C;                                                     \
+  (void)G;                                                     \
+  (void)F
+
+// Member bits of one window [lo, hi) (hi - lo <= 32) of a stencil row: bit j is
+// set when candidate lo + j is a partner of particle a, i.e. b != a and
+// r^2 < dist^2 with the reference's expression (no FMA).  Candidates are read
+// four at a time so their loads are in flight together; reads run up to 3 past
+// hi, which the position arrays' padding keeps in bounds, and those bits are
+// dropped.
+template <int D>
+__device__ __forceinline__ uint32_t window_bits(const double* const* XM, double xa0, double xa1, double xa2,
+                                                int64_t a, int32_t lo, int32_t hi, double dist2) {
+#pragma clang fp contract(off)
+  const double* p0 = XM[0] + lo;
+  const double* p1 = XM[1] + lo;
+  const double* p2 = XM[2] + lo;
+  const int32_t len = hi - lo;
+  uint32_t mk = 0;
+  for (int32_t j = 0; j < len; j += 4) {
+    double c0[4], c1[4], c2[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c0[u] = p0[j + u];
+      c1[u] = p1[j + u];
+      if (D == 3) c2[u] = p2[j + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double d0 = xa0 - c0[u];
+      const double d1 = xa1 - c1[u];
+      double r2 = d0 * d0 + d1 * d1;
+      if (D == 3) {
+        const double d2 = xa2 - c2[u];
+        r2 = r2 + d2 * d2;
+      }
+      mk |= uint32_t(r2 < dist2) << (j + u);
+    }
+  }
+  if (len < 32) mk &= (1u << len) - 1u;
+  const int64_t self = a - lo;
+  if (self >= 0 && self < len) mk &= ~(1u << self);
+  return mk;
+}
+
+// Sorted range [b, e) of stencil row k (dy = k%3 - 1, dz = k/3 - 1) of cell c;
+// empty when the row lies outside the grid.
+template <int D>
+__device__ __forceinline__ void row_range(const CellGeom& g, const int32_t* __restrict__ start, int32_t c, int k,
+                                          int32_t& b, int32_t& e) {
+  const int64_t gx = g.grid[0], gy = g.grid[1];
+  const int64_t ix = c % gx, rest = c / gx;
+  const int64_t iy = rest % gy, iz = rest / gy;
+  const int64_t xl = ix > 0 ? ix - 1 : 0, xh = ix + 1 < gx ? ix + 1 : gx - 1;
+  const int64_t y = iy + (k % 3) - 1;
+  const int64_t z = D == 3 ? iz + (k / 3) - 1 : 0;
+  const bool in = y >= 0 && y < gy && (D == 2 || (z >= 0 && z < g.grid[2]));
+  const int64_t row = gx * (y + gy * z);
+  b = in ? start[row + xl] : 0;
+  e = in ? start[row + xh + 1] : 0;
+}
+
+// Element of a per-particle array at a 32-bit byte offset: the load then uses
+// the scalar base + 32-bit vector offset addressing form (no 64-bit address
+// arithmetic per partner).  Arrays stay below 4 GiB (the problem size limit
+// checked at construction).
+template <typename T>
+__device__ __forceinline__ T ldo(const T* base, uint32_t byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+
+// Neighbour-list partners b of particle a in a fixed order (stencil rows in
+// order, ascending sorted index inside a row).  nc >= 0: the nc stored entries
+// of k_nmask, read one ahead so the next load overlaps the current window's
+// pair bodies.  nc < 0: the windows are computed here from the membership
+// positions XM.  Each lane runs its own sequence, so all lanes of a wave
+// execute the (long) pair body together.
+template <int D, typename R>
+struct Partners {
+  const SphBufs<R>& B;
+  const CellGeom& G;
+  const double* const* XM;
+  const double dist2;
+  const int64_t a;
+  const int nc;
+  const uint64_t* pe;
+  uint64_t nxt;
+  int e = 1, k = -1;
+  int32_t w0 = 0, re = 0, cb = 0;
+  uint32_t c = 0;
+
+  __device__ __forceinline__ Partners(const SphBufs<R>& B_, int64_t a_, int nc_, const CellGeom& G_,
+                                      const double* const* XM_, double dist2_)
+      : B(B_), G(G_), XM(XM_), dist2(dist2_), a(a_), nc(nc_), pe(B_.NE + a_), nxt(nc_ > 0 ? *pe : 0ull) {}
+
+  __device__ __forceinline__ bool next(int32_t& b) {
+    if (c == 0) {
+      if (nc >= 0) {
+        if (nxt == 0) return false;  // stored entries always have member bits
+        c = uint32_t(nxt);
+        cb = int32_t(nxt >> 32);
+        pe += B.n;
+        nxt = e < nc ? *pe : 0ull;
+        ++e;
+      } else {
+        while (c == 0) {
+          if (w0 >= re) {
+            if (++k >= Rows<D>::N) break;
+            row_range<D>(G, B.start, B.skey[a], k, w0, re);
+            continue;
+          }
+          const int32_t hi = re < w0 + 32 ? re : w0 + 32;
+          c = window_bits<D>(XM, XM[0][a], XM[1][a], D == 3 ? XM[2][a] : 0.0, a, w0, hi, dist2);
+          cb = w0;
+          w0 = hi;
+        }
+        if (c == 0) return false;
+      }
+    }
+    b = cb + __builtin_ctz(c);
+    c &= c - 1;
+    return true;
+  }
+};
+
+template <int D, typename R, typename F>
+__device__ __forceinline__ void for_each_partner(const SphBufs<R>& B, int64_t a, int nc, const CellGeom& G,
+                                                 const double* const* XM, double dist2, F&& body) {
+  Partners<D, R> it(B, a, nc, G, XM, dist2);
+  int32_t b;
+  while (it.next(b)) body(b);
+}
+
+// As for_each_partner, with the partner data split off: fetch(b) loads what
+// the body needs of partner b.  Pipe: the fetch of the next partner is issued
+// before the body of the current one runs, so the loads overlap the pair
+// arithmetic (at the cost of the registers holding both partners' data).
+template <bool Pipe, int D, typename R, typename Fetch, typename F>
+__device__ __forceinline__ void for_each_partner_fetch(const SphBufs<R>& B, int64_t a, int nc, const CellGeom& G,
+                                                       const double* const* XM, double dist2, Fetch&& fetch,
+                                                       F&& body) {
+  Partners<D, R> it(B, a, nc, G, XM, dist2);
+  int32_t b0, b1;
+  if constexpr (!Pipe) {
+    while (it.next(b0)) body(b0, fetch(b0));
+    return;
+  }
+  // two partners in flight, alternating between two sets of registers; a
+  // missing next partner fetches the current one again (a valid address)
+  if (!it.next(b0)) return;
+  auto f0 = fetch(b0);
+  for (;;) {
+    b1 = b0;
+    const bool m1 = it.next(b1);
+    auto f1 = fetch(b1);
+    body(b0, f0);
+    if (!m1) break;
+    b0 = b1;
+    const bool m0 = it.next(b0);
+    f0 = fetch(b0);
+    body(b1, f1);
+    if (!m0) break;
+  }
+}
+
+__device__ __forceinline__ double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+constexpr int kRedSlots = 64;
+
+// Block maximum (NaN wins) folded into one of kRedSlots slots by an atomic max
+// on its bit pattern (non-negative doubles order like their bits, NaN above
+// +Inf); spreading blocks over many slots avoids one hot address.  Every
+// thread of the block must call it.
+__device__ __forceinline__ void block_max(double v, bool active, double* slots, int blk) {
+  __shared__ double wm[kBlock / 64];
+  double m = wave_max(active ? v : 0.0);
+  if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = wm[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; ++w) r = nanmax(r, wm[w]);
+    atomic_max_nonneg(reinterpret_cast<unsigned long long*>(slots) + (blk % kRedSlots), r);
+  }
+  __syncthreads();
+}
+
+// Calls f with the scene source of L: the descriptor by value for one problem,
+// the device descriptor array and maps for several.
+template <typename R, typename Fn>
+inline void with_scenes(const LaunchSet& L, Fn&& f) {
+  if (L.one) f(OneScene<R>{*static_cast<const SceneDev<R>*>(L.one)});
+  else f(ManyScenes<R>{static_cast<const SceneDev<R>*>(L.scenes), L.pmap, L.cmap});
+}
+
+}  // namespace gclsph

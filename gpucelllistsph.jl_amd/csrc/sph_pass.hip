@@ -1,0 +1,654 @@
+// Particle-centric SPH step kernels.
+//
+// Every pair kernel of the reference (src/gpukernels_sph.jl) runs one thread per
+// *pair* and scatters both halves with FP64 atomics.  Here one thread owns one
+// particle `a` of the cell-sorted state and gathers over the 3^D cells around
+// it; the pairs it visits are exactly the reference's neighbour list (same
+// cells, r^2 < dist^2 at the positions of the last rebuild).  For the pair
+// (a, b) it evaluates the reference's per-pair expression for a's side — with
+// Δx = x_a - x_b, which is the reference's Δx or its exact negation — so the
+// per-pair values match the reference's and only the summation order differs.
+// Sums are per-thread and deterministic; there are no floating-point atomics.
+//
+// Pass 1 (tⁿ) fuses sphW!/sph∇W!, ∂ρ∂tDDT!, pressure!, ∂v∂t!, ∂v∂t_av!,
+// ∂v∂t_visc!, ∂v∂t_addgrav!, fbmolforce!, ∂v∂tpF! and the three half-step
+// updates (src/sphproblem.jl:244-274).  Pass 2 (t^{n+1/2}) fuses the second
+// evaluation with symplectic_update! and the Δt / rebuild reductions
+// (src/sphproblem.jl:277-335).
+//
+// R = double: the Float64 path.  Every expression is evaluated in the
+// reference's operation order with FMA contraction off, so branch decisions
+// (r < r0, cond < 0, clamps) see the same values as the reference would.
+// R = float: the FP32-split path (density stored as ρ-ρ0, FP32 pair terms
+// summed in FP64, Float64 positions and Δt).
+#include "sph_device.hpp"
+
+namespace gclsph {
+
+// Pass kernels come in variants that differ only in code generation (so
+// their results are bit-identical): Pipe fetches the next partner's data
+// before the current pair body (two register sets), W asks the compiler for at
+// least W waves per SIMD (fewer registers).  The step driver picks one per
+// pass by timing them on live steps.
+//
+// ---- pass 1 ---------------------------------------------------------------
+template <int D, typename R, typename SS, bool Pipe, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_pass1(SS ss) {
+#pragma clang fp contract(off)
+  GCLSPH_SCENE(ss);
+  const StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, pp = ctl->pparity, rebuilt = ctl->rebuilt;
+  const double dt = ctl->dt;
+  if (a >= B.n) return;
+  const double* X[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
+  const R* V[3] = {B.V[p][0], B.V[p][1], B.V[p][D == 3 ? 2 : 1]};
+  const R* RHO = B.RHO[p];
+  const int32_t* PT = B.PT[pp];
+  const double* XM[3];
+  membership_ptrs<D>(B, F, p, rebuilt, XM);
+
+  double xa[3];
+  R va[3];
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    xa[k] = X[k][a];
+    va[k] = V[k][a];
+  }
+  const R sa = RHO[a];
+  const int32_t pta = PT[a];
+  Sum<R> acc[3], drho;
+
+  const int nc = B.NC[a];
+  if constexpr (std::is_same<R, double>::value) {
+    const double rhoa = sa;
+    const double Pa = B.PN[a];
+    struct Nb {
+      double x[3], v[3], rho, P;
+      int32_t pt;
+    };
+    auto fetch = [&](const int32_t b) {
+      const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
+      Nb q;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        q.x[d] = ldo(X[d], o8);
+        q.v[d] = ldo(V[d], oR);
+      }
+      q.rho = ldo(RHO, oR);
+      q.P = ldo<R>(B.PN, oR);
+      q.pt = ldo(PT, o4);
+      return q;
+    };
+    for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+      const int32_t ptb = q.pt;
+      double dx[3], dv[3], gw[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        dx[d] = xa[d] - q.x[d];
+        dv[d] = va[d] - q.v[d];
+      }
+      const double r2 = dotd<D>(dx, dx);
+      const double r = sqrt(r2);
+      const double u = r * C.Hinv;
+      const double dwk_r = dW_d(u, C.dwnorm) / r;
+#pragma unroll
+      for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
+      const double rhob = q.rho;
+      const bool both = pta > 0 && ptb > 0;
+      double ax[3] = {0.0, 0.0, 0.0};
+      if (both) {  // ∂ρ∂tDDT! (:579-625)
+        double dot3 = -dotd<D>(dx, gw);
+        double s = C.m0 * dotd<D>(dv, gw);
+        if (pta > 1) {
+          double drhop = C.rho0 * pf7(1 + C.DDTgz * dx[1], C.gamma, C.ginv) - C.rho0;
+          double vd = C.DDTkh * (rhob - rhoa - drhop) / (r2 + C.eta2);
+          s += vd * dot3 * C.m0 / rhob;
+        }
+        drho.add(s);
+      }
+      if ((long long)pta * (long long)ptb > 0) {  // ∂v∂t! (:755-772)
+        double Pfac = (Pa + q.P) / (rhoa * rhob);
+#pragma unroll
+        for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
+      }
+      if (both) {  // ∂v∂t_av! (:869-897)
+        double cond = dotd<D>(dv, dx);
+        if (cond < 0) {
+          double dmu = C.h * cond / (r2 + C.eta2);
+          double dPi = (-C.alpha * C.c0 * dmu) / ((rhoa + rhob) * 0.5);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -dPi * C.m0 * gw[d];
+        }
+        if (F.nu_on) {  // ∂v∂t_visc! (:967-980)
+          double term = 4 * C.nu * C.m0 * dotd<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.eta2));
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+        }
+        if (F.s_on && r < C.st_2h) {  // ∂v∂tpF! (:1387-1397)
+          double scos = C.s * cos(C.st_c * r / C.st_2h) / (r + C.st_01h);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.m0;
+        }
+      }
+      if ((pta > 0) != (ptb > 0) && r < C.fb_r0[0]) {  // fbmolforce! (:1891-1901)
+        double q1 = C.fb_r0[0] / (r + C.fb_eta2[0]);
+        double Fc = C.fb_d[0] * (powi_d(q1, 12) - powi_d(q1, 4)) / (r2 + C.fb_eta2[0]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) ax[d] += dx[d] * Fc;
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+    });
+    // ∂v∂t_addgrav!, then the half-step updates (:270-274)
+    double av[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) av[d] = acc[d].get();
+    av[1] = av[1] - C.g;
+    double rh0, vh0[3], xh0[3];
+    if (F.synced_prev) {
+      rh0 = rhoa;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        vh0[d] = va[d];
+        xh0[d] = xa[d];
+      }
+    } else {
+      const bool mv = rebuilt;  // the reorder moved the old half-step state to XH/VH/RH
+      rh0 = mv ? B.RH[a] : B.RH2[a];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        vh0[d] = mv ? B.VH[d][a] : B.VH2[d][a];
+        xh0[d] = mv ? B.XH[d][a] : B.XH2[d][a];
+      }
+    }
+    const double hdt = dt * 0.5;
+    double rv = rh0 + drho.get() * hdt;
+    if (rv < C.rho0 && pta < 0) rv = C.rho0;
+    B.RH[a] = rv;
+    B.PH[a] = eos_d(rv, pta, C);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      double vv = vh0[d];
+      if (pta > 1) vv += av[d] * hdt;
+      B.VH[d][a] = vv;
+      B.XH[d][a] = xh0[d] + vv * hdt;
+    }
+  } else {
+    // ---------------- FP32-split
+#pragma clang fp contract(fast)
+    const float dra = sa;
+    const float rhoa = C.f_rho0 + dra;
+    const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
+    const float Pa = B.PN[a];
+    struct Nb {
+      double x[3];
+      float v[3], rho, P;
+      int32_t pt;
+    };
+    auto fetch = [&](const int32_t b) {
+      const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
+      Nb q;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        q.x[d] = ldo(X[d], o8);
+        q.v[d] = ldo(V[d], oR);
+      }
+      q.rho = ldo(RHO, oR);
+      q.P = ldo<R>(B.PN, oR);
+      q.pt = ldo(PT, o4);
+      return q;
+    };
+    for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+      const int32_t ptb = q.pt;
+      float dx[3], dv[3], gw[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        dx[d] = float(xa[d] - q.x[d]);
+        dv[d] = va[d] - q.v[d];
+      }
+      const float r2 = dotf<D>(dx, dx);
+      const float rinv = __builtin_amdgcn_rsqf(r2);
+      const float r = r2 * rinv;
+      const float u = r * C.f_Hinv;
+      const float dwk_r = dW_f(u, C.f_dwnorm) * rinv;
+      const float inv_den = __builtin_amdgcn_rcpf(r2 + C.f_eta2);
+#pragma unroll
+      for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
+      const float drb = q.rho;
+      const float rhob = C.f_rho0 + drb;
+      const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
+      const bool both = pta > 0 && ptb > 0;
+      float ax[3] = {0.0f, 0.0f, 0.0f};
+      if (both) {
+        float dot3 = -dotf<D>(dx, gw);
+        float s = C.f_m0 * dotf<D>(dv, gw);
+        if (pta > 1) {
+          float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
+          float vd = C.f_DDTkh * ((drb - dra) - drhop) * inv_den;
+          s += vd * dot3 * C.f_m0 * inv_rhob;
+        }
+        drho.add(s);
+      }
+      if ((long long)pta * (long long)ptb > 0) {
+        float Pfac = (Pa + q.P) * (inv_rhoa * inv_rhob);
+#pragma unroll
+        for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
+      }
+      if (both) {
+        float cond = dotf<D>(dv, dx);
+        if (cond < 0.0f) {
+          float dmu = C.f_h * cond * inv_den;
+          float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
+        }
+        if (F.nu_on) {
+          float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+        }
+        if (F.s_on && r < C.f_st_2h) {
+          float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] * (1.0f / C.f_m0);
+        }
+      }
+      if ((pta > 0) != (ptb > 0) && r < C.f_fb_r0[0]) {
+        // wall force in FP64: (r0/r)^12 overflows FP32 for near-coincident pairs
+        double q1 = C.fb_r0[0] / (double(r) + C.fb_eta2[0]);
+        double q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
+        double Fc = C.fb_d[0] * (q4 * q8 - q4) / (double(r2) + C.fb_eta2[0]);
+#pragma unroll
+        for (int d = 0; d < D; ++d) ax[d] += float(double(dx[d]) * Fc);
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+    });
+    float av[3];
+#pragma unroll
+    for (int d = 0; d < D; ++d) av[d] = acc[d].get();
+    av[1] = av[1] - C.f_g;
+    float rh0, vh0[3];
+    double xh0[3];
+    if (F.synced_prev) {
+      rh0 = dra;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        vh0[d] = va[d];
+        xh0[d] = xa[d];
+      }
+    } else {
+      const bool mv = rebuilt;
+      rh0 = mv ? B.RH[a] : B.RH2[a];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        vh0[d] = mv ? B.VH[d][a] : B.VH2[d][a];
+        xh0[d] = mv ? B.XH[d][a] : B.XH2[d][a];
+      }
+    }
+    const float hdt = float(dt * 0.5);
+    float rv = rh0 + drho.get() * hdt;
+    if (rv < 0.0f && pta < 0) rv = 0.0f;
+    B.RH[a] = rv;
+    B.PH[a] = eos_f(rv, pta, C);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      float vv = vh0[d];
+      if (pta > 1) vv += av[d] * hdt;
+      B.VH[d][a] = vv;
+      B.XH[d][a] = xh0[d] + double(vv) * (dt * 0.5);
+    }
+  }
+}
+
+// ---- pass 2 + symplectic update ------------------------------------------
+template <int D, typename R, typename SS, bool Pipe, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_pass2(SS ss) {
+#pragma clang fp contract(off)
+  GCLSPH_SCENE(ss);
+  StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
+  const double dt = ctl->dt;
+  const bool active = a < B.n;
+  double red_cdx = 0.0, red_acc = 0.0, red_visc = 0.0;
+  if (active) {
+    const double* X[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
+    const double* XH[3] = {B.XH[0], B.XH[1], B.XH[D == 3 ? 2 : 1]};
+    const R* VH[3] = {B.VH[0], B.VH[1], B.VH[D == 3 ? 2 : 1]};
+    const R* RH = B.RH;
+    const int32_t* PT = B.PT[pp];
+    const double* XM[3];
+    membership_ptrs<D>(B, F, p, rebuilt, XM);
+    double xa[3], xha[3];
+    R vha[3];
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      xa[k] = X[k][a];
+      xha[k] = XH[k][a];
+      vha[k] = VH[k][a];
+    }
+    const R sha = RH[a];
+    const int32_t pta = PT[a];
+    Sum<R> acc[3], drho;
+    const int nc = B.NC[a];
+    R av[3];
+    R Pha, drho_v;
+    if constexpr (std::is_same<R, double>::value) {
+      const double rhoa = sha;
+      Pha = B.PH[a];
+      struct Nb {
+        double xn[3], xh[3], vh[3], rh, P;
+        int32_t pt;
+      };
+      auto fetch = [&](const int32_t b) {
+        const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
+        Nb q;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          q.xn[d] = ldo(X[d], o8);
+          q.xh[d] = ldo(XH[d], o8);
+          q.vh[d] = ldo(VH[d], oR);
+        }
+        q.rh = ldo(RH, oR);
+        q.P = ldo<R>(B.PH, oR);
+        q.pt = ldo(PT, o4);
+        return q;
+      };
+      for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+        const int32_t ptb = q.pt;
+        double dxn[3], dx[3], dv[3], gw[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          dxn[d] = xa[d] - q.xn[d];
+          dx[d] = xha[d] - q.xh[d];
+          dv[d] = vha[d] - q.vh[d];
+        }
+        // ∇W from pass 1 (positions xⁿ, src/sphproblem.jl:246,281)
+        const double r2n = dotd<D>(dxn, dxn);
+        const double rn = sqrt(r2n);
+        const double dwk_r = dW_d(rn * C.Hinv, C.dwnorm) / rn;
+#pragma unroll
+        for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
+        const double r2 = dotd<D>(dx, dx);
+        const double rhob = q.rh;
+        const bool both = pta > 0 && ptb > 0;
+        double ax[3] = {0.0, 0.0, 0.0};
+        if (both) {
+          double dot3 = -dotd<D>(dx, gw);
+          double s = C.m0 * dotd<D>(dv, gw);
+          if (pta > 1) {
+            double drhop = C.rho0 * pf7(1 + C.DDTgz * dx[1], C.gamma, C.ginv) - C.rho0;
+            double vd = C.DDTkh * (rhob - rhoa - drhop) / (r2 + C.eta2);
+            s += vd * dot3 * C.m0 / rhob;
+          }
+          drho.add(s);
+        }
+        if ((long long)pta * (long long)ptb > 0) {
+          double Pfac = (Pha + q.P) / (rhoa * rhob);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
+        }
+        if (both) {
+          double cond = dotd<D>(dv, dx);
+          if (cond < 0) {
+            double dmu = C.h * cond / (r2 + C.eta2);
+            double dPi = (-C.alpha * C.c0 * dmu) / ((rhoa + rhob) * 0.5);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += -dPi * C.m0 * gw[d];
+          }
+          if (F.nu_on) {
+            double term = 4 * C.nu * C.m0 * dotd<D>(dx, gw) / ((rhoa + rhob) * (r2 + C.eta2));
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+          }
+          if (F.s_on) {
+            double r = sqrt(r2);
+            if (r < C.st_2h) {
+              double scos = C.s * cos(C.st_c * r / C.st_2h) / (r + C.st_01h);
+#pragma unroll
+              for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] / C.m0;
+            }
+          }
+        }
+        // fbmolforce! at xⁿ with (bound_D, bound_l) (src/sphproblem.jl:295)
+        if ((pta > 0) != (ptb > 0) && rn < C.fb_r0[1]) {
+          double q1 = C.fb_r0[1] / (rn + C.fb_eta2[1]);
+          double Fc = C.fb_d[1] * (powi_d(q1, 12) - powi_d(q1, 4)) / (r2n + C.fb_eta2[1]);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += dxn[d] * Fc;
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+      });
+#pragma unroll
+      for (int d = 0; d < D; ++d) av[d] = acc[d].get();
+      av[1] = av[1] - C.g;
+      drho_v = drho.get();
+      // symplectic_update! (src/gpukernels_sph.jl:1204-1245)
+      const double epsi = -(drho_v / rhoa) * dt;
+      double rv = B.RHO[p][a] * (2 - epsi) / (2 + epsi);
+      if (rv < C.rho0 && pta <= 1) rv = C.rho0;
+      B.RHO[q][a] = rv;
+      B.PN[a] = eos_d(rv, pta, C);
+      if (F.corr_on) B.RH2[a] = rv;
+      const double ml = pta > 1 ? 1.0 : 0.0;
+      double xn[3], vn[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        double vv = B.V[p][d][a];
+        double nv = vv + av[d] * dt * ml;
+        double dd = (vv + nv) * 0.5 * dt;
+        double cd = dd;
+        if (F.need_cdx) {
+          cd = (rebuilt ? 0.0 : B.CDX[d][a]) + dd;
+          B.CDX[d][a] = cd;
+        }
+        red_cdx = fmax(red_cdx, fabs(cd)) + (cd != cd ? cd : 0.0);
+        double xv = xa[d] + dd;
+        xn[d] = xv;
+        vn[d] = nv;
+        B.V[q][d][a] = nv;
+        B.X[q][d][a] = xv;
+        if (F.corr_on) {
+          B.VH2[d][a] = nv;
+          B.XH2[d][a] = xv;
+        }
+      }
+      if (F.ts) {
+        red_acc = sqrt(dotd<D>(av, av));
+        if (F.visc_in_pass2) red_visc = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
+      }
+    } else {
+#pragma clang fp contract(fast)
+      const float dra = sha;
+      const float rhoa = C.f_rho0 + dra;
+      const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
+      Pha = B.PH[a];
+      struct Nb {
+        double xn[3], xh[3];
+        float vh[3], rh, P;
+        int32_t pt;
+      };
+      auto fetch = [&](const int32_t b) {
+        const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
+        Nb q;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          q.xn[d] = ldo(X[d], o8);
+          q.xh[d] = ldo(XH[d], o8);
+          q.vh[d] = ldo(VH[d], oR);
+        }
+        q.rh = ldo(RH, oR);
+        q.P = ldo<R>(B.PH, oR);
+        q.pt = ldo(PT, o4);
+        return q;
+      };
+      for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+        const int32_t ptb = q.pt;
+        float dxn[3], dx[3], dv[3], gw[3];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          dxn[d] = float(xa[d] - q.xn[d]);
+          dx[d] = float(xha[d] - q.xh[d]);
+          dv[d] = vha[d] - q.vh[d];
+        }
+        const float r2n = dotf<D>(dxn, dxn);
+        const float rninv = __builtin_amdgcn_rsqf(r2n);
+        const float rn = r2n * rninv;
+        const float dwk_r = dW_f(rn * C.f_Hinv, C.f_dwnorm) * rninv;
+#pragma unroll
+        for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
+        const float r2 = dotf<D>(dx, dx);
+        const float inv_den = __builtin_amdgcn_rcpf(r2 + C.f_eta2);
+        const float drb = q.rh;
+        const float rhob = C.f_rho0 + drb;
+        const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
+        const bool both = pta > 0 && ptb > 0;
+        float ax[3] = {0.0f, 0.0f, 0.0f};
+        if (both) {
+          float dot3 = -dotf<D>(dx, gw);
+          float s = C.f_m0 * dotf<D>(dv, gw);
+          if (pta > 1) {
+            float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
+            float vd = C.f_DDTkh * ((drb - dra) - drhop) * inv_den;
+            s += vd * dot3 * C.f_m0 * inv_rhob;
+          }
+          drho.add(s);
+        }
+        if ((long long)pta * (long long)ptb > 0) {
+          float Pfac = (Pha + q.P) * (inv_rhoa * inv_rhob);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
+        }
+        if (both) {
+          float cond = dotf<D>(dv, dx);
+          if (cond < 0.0f) {
+            float dmu = C.f_h * cond * inv_den;
+            float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
+          }
+          if (F.nu_on) {
+            float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
+          }
+          if (F.s_on) {
+            float r = sqrtf(r2);
+            if (r < C.f_st_2h) {
+              float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
+#pragma unroll
+              for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] * (1.0f / C.f_m0);
+            }
+          }
+        }
+        if ((pta > 0) != (ptb > 0) && rn < C.f_fb_r0[1]) {
+          // wall force in FP64: (r0/r)^12 overflows FP32 for near-coincident pairs
+          double q1 = C.fb_r0[1] / (double(rn) + C.fb_eta2[1]);
+          double q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
+          double Fc = C.fb_d[1] * (q4 * q8 - q4) / (double(r2n) + C.fb_eta2[1]);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += float(double(dxn[d]) * Fc);
+        }
+#pragma unroll
+        for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
+      });
+#pragma unroll
+      for (int d = 0; d < D; ++d) av[d] = acc[d].get();
+      av[1] = av[1] - C.f_g;
+      drho_v = drho.get();
+      // symplectic update on δρ: ρ(2-ε)/(2+ε) - ρ0 = (2δρ - ε(2ρ0 + δρ)) / (2 + ε)
+      const float fdt = float(dt);
+      const float epsi = -(drho_v / rhoa) * fdt;
+      const float dr_n = B.RHO[p][a];
+      float rv = (2.0f * dr_n - epsi * (2.0f * C.f_rho0 + dr_n)) / (2.0f + epsi);
+      if (rv < 0.0f && pta <= 1) rv = 0.0f;
+      B.RHO[q][a] = rv;
+      B.PN[a] = eos_f(rv, pta, C);
+      if (F.corr_on) B.RH2[a] = rv;
+      const float ml = pta > 1 ? 1.0f : 0.0f;
+      double xn[3], vn[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        float vv = B.V[p][d][a];
+        float nv = vv + av[d] * fdt * ml;
+        double dd = (double(vv) + double(nv)) * 0.5 * dt;
+        double cd = dd;
+        if (F.need_cdx) {
+          cd = (rebuilt ? 0.0 : B.CDX[d][a]) + dd;
+          B.CDX[d][a] = cd;
+        }
+        red_cdx = fmax(red_cdx, fabs(cd)) + (cd != cd ? cd : 0.0);
+        double xv = xa[d] + dd;
+        xn[d] = xv;
+        vn[d] = nv;
+        B.V[q][d][a] = nv;
+        B.X[q][d][a] = xv;
+        if (F.corr_on) {
+          B.VH2[d][a] = nv;
+          B.XH2[d][a] = xv;
+        }
+      }
+      if (F.ts) {
+        float a2 = dotf<D>(av, av);
+        red_acc = sqrt(double(a2));
+        if (F.visc_in_pass2) red_visc = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
+      }
+    }
+    if (F.want_out) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) B.ACC[d][a] = av[d];
+      B.DRHO[a] = drho_v;
+      B.PBUF[a] = Pha;
+    }
+  }
+  block_max(red_cdx, active, B.RED, m.y);
+  if (F.ts) {
+    block_max(red_acc, active, B.RED + kRedSlots, m.y);
+    if (F.visc_in_pass2) block_max(red_visc, active, B.RED + 2 * kRedSlots, m.y);
+  }
+}
+
+// ------------------------------------------------------------ host launchers
+namespace {
+template <int D, typename R, bool Pipe, int W>
+void pass1_as(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) { k_pass1<D, R, decltype(ss), Pipe, W><<<L.pblocks, kBlock, 0, s>>>(ss); });
+}
+template <int D, typename R, bool Pipe, int W>
+void pass2_as(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) { k_pass2<D, R, decltype(ss), Pipe, W><<<L.pblocks, kBlock, 0, s>>>(ss); });
+}
+}  // namespace
+
+// variant: 0 pipelined; 1 not pipelined; 2 pipelined, >= 3 waves/SIMD;
+// 3 not pipelined, >= 5 waves/SIMD
+template <int D, typename R>
+void launch_pass1(const LaunchSet& L, hipStream_t s, int variant) {
+  switch (variant) {
+    case 1: pass1_as<D, R, false, 1>(L, s); break;
+    case 2: pass1_as<D, R, true, 3>(L, s); break;
+    case 3: pass1_as<D, R, false, 5>(L, s); break;
+    default: pass1_as<D, R, true, 1>(L, s); break;
+  }
+}
+template <int D, typename R>
+void launch_pass2(const LaunchSet& L, hipStream_t s, int variant) {
+  switch (variant) {
+    case 1: pass2_as<D, R, false, 1>(L, s); break;
+    case 2: pass2_as<D, R, true, 3>(L, s); break;
+    case 3: pass2_as<D, R, false, 5>(L, s); break;
+    default: pass2_as<D, R, true, 1>(L, s); break;
+  }
+}
+
+#define GCLSPH_INSTANTIATE_PASS(D, R)                                     \
+  template void launch_pass1<D, R>(const LaunchSet&, hipStream_t, int);  \
+  template void launch_pass2<D, R>(const LaunchSet&, hipStream_t, int);
+
+GCLSPH_INSTANTIATE_PASS(2, double)
+GCLSPH_INSTANTIATE_PASS(3, double)
+GCLSPH_INSTANTIATE_PASS(2, float)
+GCLSPH_INSTANTIATE_PASS(3, float)
+
+}  // namespace gclsph

@@ -329,3 +329,27 @@ def test_config2_3d_f32split_vs_f64_500_steps(G, S):
     assert er < 1e-5
     assert ev < 1e-4
     assert t32 == pytest.approx(t64, rel=1e-6)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32split"])
+def test_pass_variants_bit_identical(G, monkeypatch, prec):
+    """The pair passes exist in code-generation variants (partner prefetch,
+    occupancy target) that the step driver times on live steps and picks from;
+    every variant, forced or autotuned, must give bit-identical steps."""
+    sc, steps, ts = golden_cases()["dam2d_allphys"]
+
+    def run(variant):
+        if variant is None:
+            monkeypatch.delenv("GCLSPH_PASS_VARIANT", raising=False)
+        else:
+            monkeypatch.setenv("GCLSPH_PASS_VARIANT", variant)
+        cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, v=sc.v, precision=prec)
+        out = [G.stepsolve_(prob, 9, timestepping=ts), G.stepsolve_(prob, 3, timestepping=ts)]
+        return out + [np.asarray(gpu_fields(G, prob)[f]) for f in FIELDS]
+
+    ref = run("0,0")
+    for v in ("1,1", "2,2", "3,3", "1,3", None):
+        got = run(v)
+        for a, b in zip(got, ref):
+            assert np.array_equal(np.asarray(a), np.asarray(b)), v
