@@ -97,7 +97,36 @@ struct SphBufs {
   uint64_t* NE;
   int32_t* NC;
   int necap;
+  // FP32-split only: partner records of the two pair passes (PartnerN /
+  // PartnerH below), one per sorted particle; null in the FP64 path
+  void* PRN;
+  void* PRH;
 };
+
+// What a pair pass reads of a partner, packed into one 16-byte-aligned record
+// so that a partner costs a few 16-byte loads instead of one load per field.
+// PartnerN: the state at tⁿ read by pass 1 — written by the reorder, by pass 2
+// for the next step and by the corrections that change x, v or ρ.
+// PartnerH: what pass 2 reads — positions at tⁿ (for ∇W) and the half-step
+// state — written by pass 1.  The pressure is not stored: it is the EOS of
+// (rho, pt), recomputed with the same function that fills PN / PH.
+template <int D>
+struct alignas(16) PartnerN {
+  double x[D];
+  float v[D];
+  float rho;  // ρ - ρ0
+  int32_t pt;
+};
+template <int D>
+struct alignas(16) PartnerH {
+  double xn[D];
+  double xh[D];
+  float vh[D];
+  float rh;  // ρ½ - ρ0
+  int32_t pt;
+};
+static_assert(sizeof(PartnerN<2>) == 32 && sizeof(PartnerN<3>) == 48, "partner record sizes");
+static_assert(sizeof(PartnerH<2>) == 48 && sizeof(PartnerH<3>) == 80, "partner record sizes");
 
 // Device descriptor of one problem: everything a kernel needs, read by each
 // block from the scene it works on.

@@ -306,6 +306,34 @@ __device__ __forceinline__ T ldo(const T* base, uint32_t byte_off) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
+// Record b of a partner-record array as 16-byte loads (64-bit offsets: the
+// records are larger than the 8 bytes the 32-bit offsets above are sized for).
+template <typename T>
+__device__ __forceinline__ T ld_rec(const void* base, int64_t b) {
+  static_assert(sizeof(T) % 16 == 0, "records are whole 16-byte words");
+  const uint4* p = reinterpret_cast<const uint4*>(static_cast<const char*>(base) + b * int64_t(sizeof(T)));
+  uint4 w[sizeof(T) / 16];
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 16); ++i) w[i] = p[i];
+  T r;
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void st_rec(void* base, int64_t a, const T& r) {
+  uint4 w[sizeof(T) / 16];
+  __builtin_memcpy(w, &r, sizeof(T));
+  uint4* p = reinterpret_cast<uint4*>(static_cast<char*>(base) + a * int64_t(sizeof(T)));
+#pragma unroll
+  for (int i = 0; i < int(sizeof(T) / 16); ++i) p[i] = w[i];
+}
+
+template <int D>
+__device__ __forceinline__ PartnerN<D>* rec_n(void* base) {
+  return static_cast<PartnerN<D>*>(base);
+}
+
 // Neighbour-list partners b of particle a in a fixed order (stencil rows in
 // order, ascending sorted index inside a row).  nc >= 0: the nc stored entries
 // of k_nmask, read one ahead so the next load overlaps the current window's

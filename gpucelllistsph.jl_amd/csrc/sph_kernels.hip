@@ -162,11 +162,16 @@ __global__ void k_reorder(SS ss) {
   if (s >= B.n) return;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, pq = pp ^ 1;
   const int32_t src = B.perm[s];
+  double xs[3];
+  R vs[3];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    double xv = B.X[p][k][src];
+    const double xv = B.X[p][k][src];
+    const R vv = B.V[p][k][src];
+    xs[k] = xv;
+    vs[k] = vv;
     B.X[q][k][s] = xv;
-    B.V[q][k][s] = B.V[p][k][src];
+    B.V[q][k][s] = vv;
     if (F.write_xrb) B.XRB[k][s] = xv;
     if (!F.synced_prev) {  // previous half-step state moves with its particle
       B.XH[k][s] = B.XH2[k][src];
@@ -176,8 +181,20 @@ __global__ void k_reorder(SS ss) {
   const R rv = B.RHO[p][src];
   const int32_t pt = B.PT[pp][src];
   B.RHO[q][s] = rv;
-  if constexpr (std::is_same<R, double>::value) B.PN[s] = eos_d(rv, pt, C);
-  else B.PN[s] = eos_f(rv, pt, C);
+  if constexpr (std::is_same<R, double>::value) {
+    B.PN[s] = eos_d(rv, pt, C);
+  } else {
+    B.PN[s] = eos_f(rv, pt, C);
+    PartnerN<D> rec{};
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      rec.x[k] = xs[k];
+      rec.v[k] = vs[k];
+    }
+    rec.rho = rv;
+    rec.pt = pt;
+    st_rec(B.PRN, s, rec);
+  }
   if (!F.synced_prev) B.RH[s] = B.RH2[src];
   B.PT[pq][s] = pt;
   B.ID[pq][s] = B.ID[pp][src];
@@ -322,8 +339,14 @@ __global__ void k_dpc_apply(SS ss) {
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     R dv = B.DV[d][a];
-    B.V[q][d][a] = B.V[q][d][a] + dv;
-    B.X[q][d][a] = B.X[q][d][a] + double(dv) * dt;
+    const R nv = B.V[q][d][a] + dv;
+    const double nx = B.X[q][d][a] + double(dv) * dt;
+    B.V[q][d][a] = nv;
+    B.X[q][d][a] = nx;
+    if constexpr (std::is_same<R, float>::value) {
+      rec_n<D>(B.PRN)[a].v[d] = nv;
+      rec_n<D>(B.PRN)[a].x[d] = nx;
+    }
   }
 }
 
@@ -406,7 +429,11 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
     for (int d = 0; d < D; ++d) out[d] = double(sum[d].get());
   }
 #pragma unroll
-  for (int d = 0; d < D; ++d) B.X[q][d][a] = B.X[q][d][a] + out[d] * dt;
+  for (int d = 0; d < D; ++d) {
+    const double nx = B.X[q][d][a] + out[d] * dt;
+    B.X[q][d][a] = nx;
+    if constexpr (std::is_same<R, float>::value) rec_n<D>(B.PRN)[a].x[d] = nx;
+  }
 }
 
 // cspmcorr! (src/gpukernels_sph.jl:1693-1719): ρ = Σ m0 W / Σ m0 W / ρ_b, W at xⁿ.
@@ -475,7 +502,7 @@ __global__ __launch_bounds__(256) void k_cspm_sum(SS ss) {
   B.TMP[a] = s1 / s2;  // ρ (FP64 path) or ρ - ρ0 (FP32-split); NaN without partners
 }
 
-template <typename R, typename SS>
+template <int D, typename R, typename SS>
 __global__ void k_cspm_apply(SS ss) {
   GCLSPH_SCENE(ss);
   if (!F.cspm_fire) return;
@@ -492,6 +519,7 @@ __global__ void k_cspm_apply(SS ss) {
     const float d = float(nr);
     B.RHO[q][a] = d;
     B.PN[a] = eos_f(d, pt, C);
+    rec_n<D>(B.PRN)[a].rho = d;
   }
 }
 
@@ -801,7 +829,7 @@ template <int D, typename R>
 void launch_cspm(const LaunchSet& L, hipStream_t s) {
   with_scenes<R>(L, [&](auto ss) {
     k_cspm_sum<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss);
-    k_cspm_apply<R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss);
+    k_cspm_apply<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss);
   });
 }
 template <int D, typename R>

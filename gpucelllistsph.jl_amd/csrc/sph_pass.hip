@@ -180,24 +180,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     const float rhoa = C.f_rho0 + dra;
     const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
     const float Pa = B.PN[a];
-    struct Nb {
-      double x[3];
-      float v[3], rho, P;
-      int32_t pt;
-    };
-    auto fetch = [&](const int32_t b) {
-      const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
-      Nb q;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        q.x[d] = ldo(X[d], o8);
-        q.v[d] = ldo(V[d], oR);
-      }
-      q.rho = ldo(RHO, oR);
-      q.P = ldo<R>(B.PN, oR);
-      q.pt = ldo(PT, o4);
-      return q;
-    };
+    using Nb = PartnerN<D>;
+    auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b); };
     for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
       const int32_t ptb = q.pt;
       float dx[3], dv[3], gw[3];
@@ -230,7 +214,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         drho.add(s);
       }
       if ((long long)pta * (long long)ptb > 0) {
-        float Pfac = (Pa + q.P) * (inv_rhoa * inv_rhob);
+        float Pfac = (Pa + eos_f(drb, ptb, C)) * (inv_rhoa * inv_rhob);
 #pragma unroll
         for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
       }
@@ -291,13 +275,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     if (rv < 0.0f && pta < 0) rv = 0.0f;
     B.RH[a] = rv;
     B.PH[a] = eos_f(rv, pta, C);
+    PartnerH<D> rec{};
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       float vv = vh0[d];
       if (pta > 1) vv += av[d] * hdt;
+      const double xh = xh0[d] + double(vv) * (dt * 0.5);
       B.VH[d][a] = vv;
-      B.XH[d][a] = xh0[d] + double(vv) * (dt * 0.5);
+      B.XH[d][a] = xh;
+      rec.xn[d] = xa[d];
+      rec.xh[d] = xh;
+      rec.vh[d] = vv;
     }
+    rec.rh = rv;
+    rec.pt = pta;
+    st_rec(B.PRH, a, rec);
   }
 }
 
@@ -464,25 +456,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       const float rhoa = C.f_rho0 + dra;
       const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
       Pha = B.PH[a];
-      struct Nb {
-        double xn[3], xh[3];
-        float vh[3], rh, P;
-        int32_t pt;
-      };
-      auto fetch = [&](const int32_t b) {
-        const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
-        Nb q;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          q.xn[d] = ldo(X[d], o8);
-          q.xh[d] = ldo(XH[d], o8);
-          q.vh[d] = ldo(VH[d], oR);
-        }
-        q.rh = ldo(RH, oR);
-        q.P = ldo<R>(B.PH, oR);
-        q.pt = ldo(PT, o4);
-        return q;
-      };
+      using Nb = PartnerH<D>;
+      auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b); };
       for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
         const int32_t ptb = q.pt;
         float dxn[3], dx[3], dv[3], gw[3];
@@ -516,7 +491,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
           drho.add(s);
         }
         if ((long long)pta * (long long)ptb > 0) {
-          float Pfac = (Pha + q.P) * (inv_rhoa * inv_rhob);
+          float Pfac = (Pha + eos_f(drb, ptb, C)) * (inv_rhoa * inv_rhob);
 #pragma unroll
           for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
         }
@@ -568,6 +543,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       if (F.corr_on) B.RH2[a] = rv;
       const float ml = pta > 1 ? 1.0f : 0.0f;
       double xn[3], vn[3];
+      PartnerN<D> rec{};
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         float vv = B.V[p][d][a];
@@ -584,11 +560,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         vn[d] = nv;
         B.V[q][d][a] = nv;
         B.X[q][d][a] = xv;
+        rec.x[d] = xv;
+        rec.v[d] = nv;
         if (F.corr_on) {
           B.VH2[d][a] = nv;
           B.XH2[d][a] = xv;
         }
       }
+      rec.rho = rv;
+      rec.pt = pta;
+      st_rec(B.PRN, a, rec);  // partner record of the next step's pass 1
       if (F.ts) {
         float a2 = dotf<D>(av, av);
         red_acc = sqrt(double(a2));
