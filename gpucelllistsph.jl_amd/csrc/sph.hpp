@@ -107,9 +107,13 @@ struct SphBufs {
 // so that a partner costs a few 16-byte loads instead of one load per field.
 // PartnerN: the state at tⁿ read by pass 1 — written by the reorder, by pass 2
 // for the next step and by the corrections that change x, v or ρ.
-// PartnerH: what pass 2 reads — positions at tⁿ (for ∇W) and the half-step
-// state — written by pass 1.  The pressure is not stored: it is the EOS of
-// (rho, pt), recomputed with the same function that fills PN / PH.
+// PartnerH: the half-step state read by pass 2, written by pass 1.  Pass 2
+// also needs the partner's tⁿ position (∇W at tⁿ): when the half step started
+// from xⁿ (no correction in the previous step) x½ = xⁿ + v½·Δt/2, so the tⁿ
+// separation is the x½ separation minus (v½_a - v½_b)·Δt/2, exact to FP64
+// rounding; otherwise pass 2 reads xⁿ from the position arrays.  The pressure
+// is not stored: it is the EOS of (rho, pt), recomputed with the same function
+// that fills PN / PH.
 template <int D>
 struct alignas(16) PartnerN {
   double x[D];
@@ -119,14 +123,13 @@ struct alignas(16) PartnerN {
 };
 template <int D>
 struct alignas(16) PartnerH {
-  double xn[D];
   double xh[D];
   float vh[D];
   float rh;  // ρ½ - ρ0
   int32_t pt;
 };
 static_assert(sizeof(PartnerN<2>) == 32 && sizeof(PartnerN<3>) == 48, "partner record sizes");
-static_assert(sizeof(PartnerH<2>) == 48 && sizeof(PartnerH<3>) == 80, "partner record sizes");
+static_assert(sizeof(PartnerH<2>) == 32 && sizeof(PartnerH<3>) == 48, "partner record sizes");
 
 // Device descriptor of one problem: everything a kernel needs, read by each
 // block from the scene it works on.

@@ -283,7 +283,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       const double xh = xh0[d] + double(vv) * (dt * 0.5);
       B.VH[d][a] = vv;
       B.XH[d][a] = xh;
-      rec.xn[d] = xa[d];
       rec.xh[d] = xh;
       rec.vh[d] = vv;
     }
@@ -458,14 +457,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       Pha = B.PH[a];
       using Nb = PartnerH<D>;
       auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b); };
-      for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+      // tⁿ separation from the half-step state when x½ = xⁿ + v½·Δt/2
+      const bool derive = F.synced_prev;
+      const double hdt = dt * 0.5;
+      double ua[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) ua[d] = double(vha[d]) * hdt;
+      for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t b, const Nb& q) {
         const int32_t ptb = q.pt;
         float dxn[3], dx[3], dv[3], gw[3];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          dxn[d] = float(xa[d] - q.xn[d]);
-          dx[d] = float(xha[d] - q.xh[d]);
+          const double dh = xha[d] - q.xh[d];
+          dx[d] = float(dh);
           dv[d] = vha[d] - q.vh[d];
+          dxn[d] = derive ? float(dh - (ua[d] - double(q.vh[d]) * hdt)) : float(xa[d] - X[d][b]);
         }
         const float r2n = dotf<D>(dxn, dxn);
         const float rninv = __builtin_amdgcn_rsqf(r2n);
