@@ -31,6 +31,8 @@ struct SphConst {
   float f_h, f_Hinv, f_rho0, f_m0, f_alpha, f_c0, f_g, f_s, f_nu, f_eta2, f_DDTkh, f_DDTgz, f_P0, f_wnorm,
       f_dwnorm, f_fb_d[2], f_fb_r0[2], f_fb_eta2[2], f_st_2h, f_st_c, f_st_01h, f_xsph_c;
   float f_dpc_l0inv, f_dpc_eta2, f_dpc_whinv, f_dpc_lim, f_dpc_wnorm, f_dpc_lambda, f_dpc_pmin, f_dpc_pmax;
+  float f_inv_rho0;  // 1/ρ0
+  float f_dwr;       // dW/dr / r = f_dwr (1-u)^3: -20 dwnorm H⁻¹
 };
 
 // Switches of one problem.  The host sets the per-call ones; the per-step ones

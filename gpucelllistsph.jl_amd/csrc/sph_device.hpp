@@ -83,6 +83,11 @@ __device__ __forceinline__ float dW_f(float u, float dwnorm) {
   }
   return 0.0f;
 }
+// (dW/dr)/r at u = r/H, i.e. ∇W = dx · dWr_f: -20 (1-u)^3 dwnorm / H (u/r = 1/H).
+__device__ __forceinline__ float dWr_f(float u, float dwr) {
+  const float um1 = 1.0f - u;
+  return u < 1.0f ? dwr * (um1 * um1 * um1) : 0.0f;
+}
 
 // Tait EOS (src/gpukernels_sph.jl:691-697): 0 if ptype <= 1 and ρ < ρ0.
 __device__ __forceinline__ double eos_d(double rho, int32_t pt, const SphConst& C) {
@@ -93,7 +98,7 @@ __device__ __forceinline__ double eos_d(double rho, int32_t pt, const SphConst& 
 // FP32-split EOS on δρ: P0((1+e)^7 - 1) = P0 e (7 + 21e + 35e² + 35e³ + 21e⁴ + 7e⁵ + e⁶).
 __device__ __forceinline__ float eos_f(float drho, int32_t pt, const SphConst& C) {
   if (pt <= 1 && drho < 0.0f) return 0.0f;
-  float e = drho / C.f_rho0;
+  float e = drho * C.f_inv_rho0;
   float poly = 7.0f + e * (21.0f + e * (35.0f + e * (35.0f + e * (21.0f + e * (7.0f + e)))));
   return C.f_P0 * (e * poly);
 }
@@ -246,12 +251,13 @@ __device__ __forceinline__ StepFlags step_flags(const SceneDev<R>& sd) {
 // hi, which the position arrays' padding keeps in bounds, and those bits are
 // dropped.
 template <int D>
-__device__ __forceinline__ uint32_t window_bits(const double* const* XM, double xa0, double xa1, double xa2,
-                                                int64_t a, int32_t lo, int32_t hi, double dist2) {
+__device__ __forceinline__ uint32_t window_bits(const double* x0, const double* x1, const double* x2, double xa0,
+                                                double xa1, double xa2, int64_t a, int32_t lo, int32_t hi,
+                                                double dist2) {
 #pragma clang fp contract(off)
-  const double* p0 = XM[0] + lo;
-  const double* p1 = XM[1] + lo;
-  const double* p2 = XM[2] + lo;
+  const double* p0 = x0 + lo;
+  const double* p1 = x1 + lo;
+  const double* p2 = x2 + lo;
   const int32_t len = hi - lo;
   uint32_t mk = 0;
   for (int32_t j = 0; j < len; j += 4) {
@@ -334,6 +340,34 @@ __device__ __forceinline__ PartnerN<D>* rec_n(void* base) {
   return static_cast<PartnerN<D>*>(base);
 }
 
+// The union of the stencil rows of one block (2D).  The block's consecutive
+// particles a0..a1 occupy one contiguous range of cells [c0, c1], so for each
+// row offset k the cells [c0 + (k-1)·gx - 1, c1 + (k-1)·gx + 1] hold every
+// candidate of row k of every particle of the block: three contiguous ranges
+// of sorted particles starting at ub[k], laid out one after the other from
+// LDS slot uo[k].
+struct RowUnion {
+  int32_t ub[3], uo[4];
+};
+
+__device__ __forceinline__ void row_union(RowUnion& u, const int32_t* start, const int32_t* skey, const CellGeom& G,
+                                          int64_t n, int64_t a0) {
+  const int64_t a1 = (a0 + kBlock < n ? a0 + kBlock : n) - 1;
+  const int64_t c0 = skey[a0], c1 = skey[a1];
+  const int64_t gx = G.grid[0], nc = G.ncells();
+  int32_t o = 0;
+  for (int k = 0; k < 3; ++k) {
+    int64_t lo = c0 + (k - 1) * gx - 1, hi = c1 + (k - 1) * gx + 1;
+    lo = lo < 0 ? 0 : (lo > nc - 1 ? nc - 1 : lo);
+    hi = hi < 0 ? 0 : (hi > nc - 1 ? nc - 1 : hi);
+    const int32_t b = start[lo], e = start[hi + 1];
+    u.ub[k] = b;
+    u.uo[k] = o;
+    o += e > b ? e - b : 0;
+  }
+  u.uo[3] = o;
+}
+
 // Neighbour-list partners b of particle a in a fixed order (stencil rows in
 // order, ascending sorted index inside a row).  nc >= 0: the nc stored entries
 // of k_nmask, read one ahead so the next load overlaps the current window's
@@ -344,7 +378,9 @@ template <int D, typename R>
 struct Partners {
   const SphBufs<R>& B;
   const CellGeom& G;
-  const double* const* XM;
+  const double* const xm0;  // membership positions, held by value
+  const double* const xm1;
+  const double* const xm2;
   const double dist2;
   const int64_t a;
   const int nc;
@@ -356,44 +392,68 @@ struct Partners {
 
   __device__ __forceinline__ Partners(const SphBufs<R>& B_, int64_t a_, int nc_, const CellGeom& G_,
                                       const double* const* XM_, double dist2_)
-      : B(B_), G(G_), XM(XM_), dist2(dist2_), a(a_), nc(nc_), pe(B_.NE + a_), nxt(nc_ > 0 ? *pe : 0ull) {}
+      : B(B_),
+        G(G_),
+        xm0(XM_[0]),
+        xm1(XM_[1]),
+        xm2(XM_[2]),
+        dist2(dist2_),
+        a(a_),
+        nc(nc_),
+        pe(B_.NE + a_),
+        nxt(nc_ > 0 ? *pe : 0ull) {
+    fill();
+  }
 
-  __device__ __forceinline__ bool next(int32_t& b) {
-    if (c == 0) {
-      if (nc >= 0) {
-        if (nxt == 0) return false;  // stored entries always have member bits
-        c = uint32_t(nxt);
-        cb = int32_t(nxt >> 32);
-        pe += B.n;
-        nxt = e < nc ? *pe : 0ull;
-        ++e;
-      } else {
-        while (c == 0) {
-          if (w0 >= re) {
-            if (++k >= Rows<D>::N) break;
-            row_range<D>(G, B.start, B.skey[a], k, w0, re);
-            continue;
-          }
-          const int32_t hi = re < w0 + 32 ? re : w0 + 32;
-          c = window_bits<D>(XM, XM[0][a], XM[1][a], D == 3 ? XM[2][a] : 0.0, a, w0, hi, dist2);
-          cb = w0;
-          w0 = hi;
-        }
-        if (c == 0) return false;
-      }
-    }
-    b = cb + __builtin_ctz(c);
+  // Another partner is waiting (the iterator always stands on the next one).
+  __device__ __forceinline__ bool has() const { return c != 0; }
+
+  // The waiting partner; moves on to the one after it.
+  __device__ __forceinline__ int32_t take() {
+    const int32_t b = cb + __builtin_ctz(c);
     c &= c - 1;
-    return true;
+    if (c == 0) fill();
+    return b;
+  }
+
+  // c == 0: loads the next window that has members (c stays 0 at the end).
+  __device__ __forceinline__ void fill() {
+    if (nc >= 0) {
+      if (nxt == 0) return;  // stored entries always have member bits
+      c = uint32_t(nxt);
+      cb = int32_t(nxt >> 32);
+      pe += B.n;
+      nxt = e < nc ? *pe : 0ull;
+      ++e;
+      return;
+    }
+    while (c == 0) {
+      if (w0 >= re) {
+        if (++k >= Rows<D>::N) return;
+        row_range<D>(G, B.start, B.skey[a], k, w0, re);
+        continue;
+      }
+      const int32_t hi = re < w0 + 32 ? re : w0 + 32;
+      c = window_bits<D>(xm0, xm1, xm2, xm0[a], xm1[a], D == 3 ? xm2[a] : 0.0, a, w0, hi, dist2);
+      cb = w0;
+      w0 = hi;
+    }
   }
 };
 
+// The partner loops below leave together: a lane without partners left sits
+// out the remaining trips (masked off), and the loop ends when no lane of the
+// wave has one.  A loop whose lanes leave one by one makes the compiler carry
+// every loop value in a second set of registers with copies on each trip.
 template <int D, typename R, typename F>
 __device__ __forceinline__ void for_each_partner(const SphBufs<R>& B, int64_t a, int nc, const CellGeom& G,
                                                  const double* const* XM, double dist2, F&& body) {
   Partners<D, R> it(B, a, nc, G, XM, dist2);
-  int32_t b;
-  while (it.next(b)) body(b);
+  for (;;) {
+    const bool m = it.has();
+    if (!__any(m)) break;
+    if (m) body(it.take());
+  }
 }
 
 // As for_each_partner, with the partner data split off: fetch(b) loads what
@@ -405,26 +465,35 @@ __device__ __forceinline__ void for_each_partner_fetch(const SphBufs<R>& B, int6
                                                        const double* const* XM, double dist2, Fetch&& fetch,
                                                        F&& body) {
   Partners<D, R> it(B, a, nc, G, XM, dist2);
-  int32_t b0, b1;
   if constexpr (!Pipe) {
-    while (it.next(b0)) body(b0, fetch(b0));
+    for (;;) {
+      const bool m = it.has();
+      if (!__any(m)) break;
+      if (m) {
+        const int32_t b = it.take();
+        body(b, fetch(b));
+      }
+    }
     return;
   }
-  // two partners in flight, alternating between two sets of registers; a
-  // missing next partner fetches the current one again (a valid address)
-  if (!it.next(b0)) return;
+  // two partners in flight, alternating between two sets of registers; a lane
+  // without a next partner fetches its previous one (or itself) again, which
+  // is a valid address, and skips the body
+  bool m0 = it.has();
+  if (!__any(m0)) return;
+  int32_t b0 = m0 ? it.take() : int32_t(a);
   auto f0 = fetch(b0);
   for (;;) {
-    b1 = b0;
-    const bool m1 = it.next(b1);
+    const bool m1 = it.has();
+    const int32_t b1 = m1 ? it.take() : b0;
     auto f1 = fetch(b1);
-    body(b0, f0);
-    if (!m1) break;
-    b0 = b1;
-    const bool m0 = it.next(b0);
+    if (m0) body(b0, f0);
+    if (!__any(m1)) break;
+    m0 = it.has();
+    b0 = m0 ? it.take() : b1;
     f0 = fetch(b0);
-    body(b1, f1);
-    if (!m0) break;
+    if (m1) body(b1, f1);
+    if (!__any(m0)) break;
   }
 }
 

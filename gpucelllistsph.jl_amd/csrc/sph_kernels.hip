@@ -44,9 +44,7 @@ constexpr int kNmaskLds = 2048;
 // entry count, or -1 when the particle needs more than necap entries (its
 // partners are then found by scanning the stencil).
 //
-// 2D: the block's 256 consecutive particles lie in one contiguous range of
-// cells, so the union of their stencil rows is, per row offset, one contiguous
-// range of sorted particles.  The block stages those positions in LDS once
+// 2D: the block stages the positions of its row union (row_union) in LDS once
 // (coalesced) and every lane tests its candidates there, instead of each lane
 // fetching each of its ~100 candidates through the vector L1.  A block whose
 // union exceeds the LDS budget reads the candidates from global memory.
@@ -65,24 +63,11 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
   uint64_t* out = B.NE + a;
   if constexpr (D == 2) {
     __shared__ double2 lds[kNmaskLds + 4];  // +4: window reads run up to 3 past a row's end
-    __shared__ int32_t ub[3], uo[4];
-    if (threadIdx.x == 0) {
-      const int64_t a1 = (a0 + kBlock < B.n ? a0 + kBlock : B.n) - 1;
-      const int64_t c0 = B.skey[a0], c1 = B.skey[a1];
-      const int64_t gx = G.grid[0], nc = G.ncells();
-      int32_t o = 0;
-      for (int k = 0; k < 3; ++k) {
-        int64_t lo = c0 + (k - 1) * gx - 1, hi = c1 + (k - 1) * gx + 1;
-        lo = lo < 0 ? 0 : (lo > nc - 1 ? nc - 1 : lo);
-        hi = hi < 0 ? 0 : (hi > nc - 1 ? nc - 1 : hi);
-        const int32_t b = B.start[lo], e = B.start[hi + 1];
-        ub[k] = b;
-        uo[k] = o;
-        o += e > b ? e - b : 0;
-      }
-      uo[3] = o;
-    }
+    __shared__ RowUnion u;
+    if (threadIdx.x == 0) row_union(u, B.start, B.skey, G, B.n, a0);
     __syncthreads();
+    const int32_t* ub = u.ub;
+    const int32_t* uo = u.uo;
     const int32_t total = uo[3];
     if (total <= kNmaskLds) {  // uniform over the block
       for (int32_t i = threadIdx.x; i < total; i += kBlock) {
@@ -119,7 +104,7 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
   for (int k = 0; k < Rows<D>::N; ++k) {
     for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
       const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
-      const uint32_t mk = window_bits<D>(XM, xa0, xa1, xa2, a, w0, hi, dist2);
+      const uint32_t mk = window_bits<D>(XM[0], XM[1], XM[2], xa0, xa1, xa2, a, w0, hi, dist2);
       if (mk != 0) {
         if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
         ++cnt;

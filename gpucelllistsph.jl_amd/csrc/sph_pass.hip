@@ -179,64 +179,57 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     const float dra = sa;
     const float rhoa = C.f_rho0 + dra;
     const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
+    const float mPa = -C.f_m0 * inv_rhoa;  // -m0/ρ_a of the pressure term
     const float Pa = B.PN[a];
     using Nb = PartnerN<D>;
     auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b); };
+    // ∇W = dx·dwr, so every term along ∇W is gathered into one coefficient fg
+    // of dx, and dx·∇W = r²·dwr, dv·∇W = (dv·dx)·dwr.
     for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
       const int32_t ptb = q.pt;
-      float dx[3], dv[3], gw[3];
+      float dx[3], dv[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         dx[d] = float(xa[d] - q.x[d]);
         dv[d] = va[d] - q.v[d];
       }
       const float r2 = dotf<D>(dx, dx);
-      const float rinv = __builtin_amdgcn_rsqf(r2);
-      const float r = r2 * rinv;
-      const float u = r * C.f_Hinv;
-      const float dwk_r = dW_f(u, C.f_dwnorm) * rinv;
+      const float r = __builtin_amdgcn_sqrtf(r2);
+      const float dwr = dWr_f(r * C.f_Hinv, C.f_dwr);
       const float inv_den = __builtin_amdgcn_rcpf(r2 + C.f_eta2);
-#pragma unroll
-      for (int d = 0; d < D; ++d) gw[d] = dx[d] * dwk_r;
+      const float dvdx = dotf<D>(dv, dx);
       const float drb = q.rho;
       const float rhob = C.f_rho0 + drb;
       const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
+      // The common terms are evaluated for every pair and selected (no
+      // divergent branches in the hot loop): continuity + δ-SPH, pressure, AV.
       const bool both = pta > 0 && ptb > 0;
       float ax[3] = {0.0f, 0.0f, 0.0f};
-      if (both) {
-        float dot3 = -dotf<D>(dx, gw);
-        float s = C.f_m0 * dotf<D>(dv, gw);
-        if (pta > 1) {
-          float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
-          float vd = C.f_DDTkh * ((drb - dra) - drhop) * inv_den;
-          s += vd * dot3 * C.f_m0 * inv_rhob;
-        }
-        drho.add(s);
+      {
+        const float s = C.f_m0 * dwr * dvdx;
+        const float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
+        const float vd = C.f_DDTkh * ((drb - dra) - drhop) * inv_den;
+        const float sd = s - vd * (r2 * dwr) * C.f_m0 * inv_rhob;
+        drho.add(both ? (pta > 1 ? sd : s) : 0.0f);
       }
-      if ((long long)pta * (long long)ptb > 0) {
-        float Pfac = (Pa + eos_f(drb, ptb, C)) * (inv_rhoa * inv_rhob);
-#pragma unroll
-        for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
+      const float Pt = (Pa + eos_f(drb, ptb, C)) * inv_rhob * mPa;
+      float fg = (long long)pta * (long long)ptb > 0 ? Pt : 0.0f;
+      {
+        const float dmu = C.f_h * fminf(dvdx, 0.0f) * inv_den;  // AV acts on approaching pairs only
+        const float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
+        fg -= both ? dPi * C.f_m0 : 0.0f;
       }
       if (both) {
-        float cond = dotf<D>(dv, dx);
-        if (cond < 0.0f) {
-          float dmu = C.f_h * cond * inv_den;
-          float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
-#pragma unroll
-          for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
-        }
         if (F.nu_on) {
-          float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
+          float term = 4.0f * C.f_nu * C.f_m0 * (r2 * dwr) * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
 #pragma unroll
           for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
         }
-        if (F.s_on && r < C.f_st_2h) {
-          float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
-#pragma unroll
-          for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] * (1.0f / C.f_m0);
-        }
       }
+      fg *= dwr;
+      if (both && F.s_on && r < C.f_st_2h) fg += C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h) * (1.0f / C.f_m0);
+#pragma unroll
+      for (int d = 0; d < D; ++d) ax[d] += fg * dx[d];
       if ((pta > 0) != (ptb > 0) && r < C.f_fb_r0[0]) {
         // wall force in FP64: (r0/r)^12 overflows FP32 for near-coincident pairs
         double q1 = C.fb_r0[0] / (double(r) + C.fb_eta2[0]);
@@ -454,6 +447,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       const float dra = sha;
       const float rhoa = C.f_rho0 + dra;
       const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
+      const float mPa = -C.f_m0 * inv_rhoa;  // -m0/ρ_a of the pressure term
       Pha = B.PH[a];
       using Nb = PartnerH<D>;
       auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b); };
@@ -465,7 +459,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       for (int d = 0; d < D; ++d) ua[d] = double(vha[d]) * hdt;
       for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t b, const Nb& q) {
         const int32_t ptb = q.pt;
-        float dxn[3], dx[3], dv[3], gw[3];
+        float dxn[3], dx[3], dv[3];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           const double dh = xha[d] - q.xh[d];
@@ -473,44 +467,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
           dv[d] = vha[d] - q.vh[d];
           dxn[d] = derive ? float(dh - (ua[d] - double(q.vh[d]) * hdt)) : float(xa[d] - X[d][b]);
         }
+        // ∇W from pass 1 (positions xⁿ): ∇W = dxn·dwr; terms along ∇W are
+        // gathered into one coefficient fg of dxn
         const float r2n = dotf<D>(dxn, dxn);
-        const float rninv = __builtin_amdgcn_rsqf(r2n);
-        const float rn = r2n * rninv;
-        const float dwk_r = dW_f(rn * C.f_Hinv, C.f_dwnorm) * rninv;
-#pragma unroll
-        for (int d = 0; d < D; ++d) gw[d] = dxn[d] * dwk_r;
+        const float rn = __builtin_amdgcn_sqrtf(r2n);
+        const float dwr = dWr_f(rn * C.f_Hinv, C.f_dwr);
         const float r2 = dotf<D>(dx, dx);
         const float inv_den = __builtin_amdgcn_rcpf(r2 + C.f_eta2);
+        const float dvdx = dotf<D>(dv, dx);
         const float drb = q.rh;
         const float rhob = C.f_rho0 + drb;
         const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
+        // common terms evaluated for every pair and selected, as in pass 1
         const bool both = pta > 0 && ptb > 0;
         float ax[3] = {0.0f, 0.0f, 0.0f};
-        if (both) {
-          float dot3 = -dotf<D>(dx, gw);
-          float s = C.f_m0 * dotf<D>(dv, gw);
-          if (pta > 1) {
-            float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
-            float vd = C.f_DDTkh * ((drb - dra) - drhop) * inv_den;
-            s += vd * dot3 * C.f_m0 * inv_rhob;
-          }
-          drho.add(s);
-        }
-        if ((long long)pta * (long long)ptb > 0) {
-          float Pfac = (Pha + eos_f(drb, ptb, C)) * (inv_rhoa * inv_rhob);
-#pragma unroll
-          for (int d = 0; d < D; ++d) ax[d] += -C.f_m0 * Pfac * gw[d];
+        const float dxg = dotf<D>(dx, dxn) * dwr;  // dx·∇W
+        {
+          const float s = C.f_m0 * dwr * dotf<D>(dv, dxn);
+          const float drhop = hydro_drho_f(C.f_DDTgz * dx[1], C.f_rho0);
+          const float vd = C.f_DDTkh * ((drb - dra) - drhop) * inv_den;
+          const float sd = s - vd * dxg * C.f_m0 * inv_rhob;
+          drho.add(both ? (pta > 1 ? sd : s) : 0.0f);
         }
         if (both) {
-          float cond = dotf<D>(dv, dx);
-          if (cond < 0.0f) {
-            float dmu = C.f_h * cond * inv_den;
-            float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
-#pragma unroll
-            for (int d = 0; d < D; ++d) ax[d] += -dPi * C.f_m0 * gw[d];
-          }
           if (F.nu_on) {
-            float term = 4.0f * C.f_nu * C.f_m0 * dotf<D>(dx, gw) * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
+            float term = 4.0f * C.f_nu * C.f_m0 * dxg * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
 #pragma unroll
             for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
           }
@@ -523,6 +504,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
             }
           }
         }
+        const float Pt = (Pha + eos_f(drb, ptb, C)) * inv_rhob * mPa;
+        float fg = (long long)pta * (long long)ptb > 0 ? Pt : 0.0f;
+        {
+          const float dmu = C.f_h * fminf(dvdx, 0.0f) * inv_den;
+          const float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
+          fg -= both ? dPi * C.f_m0 : 0.0f;
+        }
+        fg *= dwr;
+#pragma unroll
+        for (int d = 0; d < D; ++d) ax[d] += fg * dxn[d];
         if ((pta > 0) != (ptb > 0) && rn < C.f_fb_r0[1]) {
           // wall force in FP64: (r0/r)^12 overflows FP32 for near-coincident pairs
           double q1 = C.fb_r0[1] / (double(rn) + C.fb_eta2[1]);
