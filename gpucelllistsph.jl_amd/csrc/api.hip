@@ -294,14 +294,10 @@ struct sph_s {
     B.NC = dalloc<int32_t>(n);
     B.NE = nullptr;
     B.necap = 0;
-    B.PRN = nullptr;
-    B.PRH = nullptr;
-    if (std::is_same<R, float>::value) {
-      const size_t rn = dim == 2 ? sizeof(PartnerN<2>) : sizeof(PartnerN<3>);
-      const size_t rh = dim == 2 ? sizeof(PartnerH<2>) : sizeof(PartnerH<3>);
-      B.PRN = dalloc<uint8_t>(int64_t(rn) * n);
-      B.PRH = dalloc<uint8_t>(int64_t(rh) * n);
-    }
+    const size_t rn = dim == 2 ? sizeof(typename Records<2, R>::N) : sizeof(typename Records<3, R>::N);
+    const size_t rh = dim == 2 ? sizeof(typename Records<2, R>::H) : sizeof(typename Records<3, R>::H);
+    B.PRN = dalloc<uint8_t>(int64_t(rn) * n);
+    B.PRH = dalloc<uint8_t>(int64_t(rh) * n);
   }
 
   // Entry store of k_nmask: a particle needs at most one entry per 32-candidate

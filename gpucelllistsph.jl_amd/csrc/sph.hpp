@@ -99,8 +99,8 @@ struct SphBufs {
   uint64_t* NE;
   int32_t* NC;
   int necap;
-  // FP32-split only: partner records of the two pair passes (PartnerN /
-  // PartnerH below), one per sorted particle; null in the FP64 path
+  // partner records of the pair kernels, one per sorted particle: PartnerN /
+  // PartnerH(X) below in FP32-split, PartnerN64 / PartnerH64 in FP64
   void* PRN;
   void* PRH;
 };
@@ -113,9 +113,10 @@ struct SphBufs {
 // also needs the partner's tⁿ position (∇W at tⁿ): when the half step started
 // from xⁿ (no correction in the previous step) x½ = xⁿ + v½·Δt/2, so the tⁿ
 // separation is the x½ separation minus (v½_a - v½_b)·Δt/2, exact to FP64
-// rounding; otherwise pass 2 reads xⁿ from the position arrays.  The pressure
-// is not stored: it is the EOS of (rho, pt), recomputed with the same function
-// that fills PN / PH.
+// rounding; otherwise the records are PartnerHX, which append xⁿ (the step
+// flag synced_prev tells both passes which layout the step uses).  The
+// pressure is not stored: it is the EOS of (rho, pt), recomputed with the same
+// function that fills PN / PH.
 template <int D>
 struct alignas(16) PartnerN {
   double x[D];
@@ -131,7 +132,51 @@ struct alignas(16) PartnerH {
   int32_t pt;
 };
 static_assert(sizeof(PartnerN<2>) == 32 && sizeof(PartnerN<3>) == 48, "partner record sizes");
+template <int D>
+struct alignas(16) PartnerHX {
+  PartnerH<D> h;
+  double xn[D];
+};
 static_assert(sizeof(PartnerH<2>) == 32 && sizeof(PartnerH<3>) == 48, "partner record sizes");
+static_assert(sizeof(PartnerHX<2>) == 48 && sizeof(PartnerHX<3>) == 80, "partner record sizes");
+
+// FP64 path: the pressure is stored (its EOS costs an FP64 division and a
+// power per partner), and pass 2's record always carries xⁿ (an FP64-exact
+// tⁿ separation cannot be derived from the half-step state).
+template <int D>
+struct alignas(16) PartnerN64 {
+  double x[D];
+  double v[D];
+  double rho;
+  double P;
+  int32_t pt;
+};
+template <int D>
+struct alignas(16) PartnerH64 {
+  double xn[D];
+  double xh[D];
+  double vh[D];
+  double rh;
+  double P;
+  int32_t pt;
+};
+static_assert(sizeof(PartnerN64<2>) == 64 && sizeof(PartnerN64<3>) == 80, "partner record sizes");
+static_assert(sizeof(PartnerH64<2>) == 80 && sizeof(PartnerH64<3>) == 96, "partner record sizes");
+
+// Record types of a precision: N (state at tⁿ, pass 1 and the corrections) and
+// H (pass 2; FP32-split uses PartnerH or PartnerHX by step, see above).
+template <int D, typename R>
+struct Records;
+template <int D>
+struct Records<D, float> {
+  using N = PartnerN<D>;
+  using H = PartnerHX<D>;
+};
+template <int D>
+struct Records<D, double> {
+  using N = PartnerN64<D>;
+  using H = PartnerH64<D>;
+};
 
 // Device descriptor of one problem: everything a kernel needs, read by each
 // block from the scene it works on.

@@ -335,9 +335,9 @@ __device__ __forceinline__ void st_rec(void* base, int64_t a, const T& r) {
   for (int i = 0; i < int(sizeof(T) / 16); ++i) p[i] = w[i];
 }
 
-template <int D>
-__device__ __forceinline__ PartnerN<D>* rec_n(void* base) {
-  return static_cast<PartnerN<D>*>(base);
+template <int D, typename R>
+__device__ __forceinline__ typename Records<D, R>::N* rec_n(void* base) {
+  return static_cast<typename Records<D, R>::N*>(base);
 }
 
 // The union of the stencil rows of one block (2D).  The block's consecutive

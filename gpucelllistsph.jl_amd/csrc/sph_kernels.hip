@@ -166,20 +166,22 @@ __global__ void k_reorder(SS ss) {
   const R rv = B.RHO[p][src];
   const int32_t pt = B.PT[pp][src];
   B.RHO[q][s] = rv;
+  typename Records<D, R>::N rec{};
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    rec.x[k] = xs[k];
+    rec.v[k] = vs[k];
+  }
+  rec.rho = rv;
+  rec.pt = pt;
   if constexpr (std::is_same<R, double>::value) {
-    B.PN[s] = eos_d(rv, pt, C);
+    const double P = eos_d(rv, pt, C);
+    B.PN[s] = P;
+    rec.P = P;
   } else {
     B.PN[s] = eos_f(rv, pt, C);
-    PartnerN<D> rec{};
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-      rec.x[k] = xs[k];
-      rec.v[k] = vs[k];
-    }
-    rec.rho = rv;
-    rec.pt = pt;
-    st_rec(B.PRN, s, rec);
   }
+  st_rec(B.PRN, s, rec);
   if (!F.synced_prev) B.RH[s] = B.RH2[src];
   B.PT[pq][s] = pt;
   B.ID[pq][s] = B.ID[pp][src];
@@ -211,24 +213,9 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
     xa[d] = X[d][a];
     va[d] = V[d][a];
   }
-  struct Nb {
-    double x[3];
-    R v[3], rho, P;
-    int32_t pt;
-  };
-  auto fetch = [&](const int32_t b) {
-    const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
-    Nb n;
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      n.x[d] = ldo(X[d], o8);
-      n.v[d] = ldo(V[d], oR);
-    }
-    n.rho = ldo(RHO, oR);
-    n.P = ldo<R>(B.PN, oR);
-    n.pt = ldo(PT, o4);
-    return n;
-  };
+  // partners from the records, which hold the updated state (pass 2)
+  using Nb = typename Records<D, R>::N;
+  auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b); };
   const R Pa = B.PN[a];
   if (F.want_out) B.PBUF[a] = Pa;
   if (pta <= 0) {  // every DPC pair needs ptype > 0 on both sides
@@ -290,7 +277,7 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
       const float w = W_f(u, C.f_dpc_wnorm);
       const float chi = __builtin_amdgcn_sqrtf(w * C.f_dpc_whinv);
       const float kk = u < C.f_dpc_lim ? 1.0f : chi;
-      const float lo = C.f_dpc_lambda * fabsf(Pa + n.P), cap = C.f_dpc_lambda * C.f_dpc_pmax;
+      const float lo = C.f_dpc_lambda * fabsf(Pa + eos_f(n.rho, n.pt, C)), cap = C.f_dpc_lambda * C.f_dpc_pmax;
       float pt = lo < cap ? lo : cap;
       pt = pt > C.f_dpc_pmin ? pt : C.f_dpc_pmin;
       const float Pb_dpc = chi * pt;
@@ -328,10 +315,8 @@ __global__ void k_dpc_apply(SS ss) {
     const double nx = B.X[q][d][a] + double(dv) * dt;
     B.V[q][d][a] = nv;
     B.X[q][d][a] = nx;
-    if constexpr (std::is_same<R, float>::value) {
-      rec_n<D>(B.PRN)[a].v[d] = nv;
-      rec_n<D>(B.PRN)[a].x[d] = nx;
-    }
+    rec_n<D, R>(B.PRN)[a].v[d] = nv;
+    rec_n<D, R>(B.PRN)[a].x[d] = nx;
   }
 }
 
@@ -361,32 +346,29 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
     xa[d] = XN[d][a];
     va[d] = V[d][a];
   }
+  // v, ρ and ptype of a partner from its record (updated by the DPC), xⁿ from
+  // the position arrays
   struct Nb {
-    double x[3];
-    R v[3], rho;
-    int32_t pt;
+    typename Records<D, R>::N r;
+    double xn[3];
   };
   auto fetch = [&](const int32_t b) {
-    const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
     Nb n;
+    n.r = ld_rec<typename Records<D, R>::N>(B.PRN, b);
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      n.x[d] = ldo(XN[d], o8);
-      n.v[d] = ldo(V[d], oR);
-    }
-    n.rho = ldo(RHO, oR);
-    n.pt = ldo(PT, o4);
+    for (int d = 0; d < D; ++d) n.xn[d] = XN[d][b];
     return n;
   };
   double out[3];
   if constexpr (std::is_same<R, double>::value) {
     const double rhoa = RHO[a];
     double sum[3] = {0.0, 0.0, 0.0};
-    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& m) {
+      const auto& n = m.r;
       if (n.pt <= 1) return;
       double dx[3];
 #pragma unroll
-      for (int d = 0; d < D; ++d) dx[d] = xa[d] - n.x[d];
+      for (int d = 0; d < D; ++d) dx[d] = xa[d] - m.xn[d];
       const double r = sqrt(dotd<D>(dx, dx));
       const double w = W_d(r * C.Hinv, C.wnorm);
       const double xs = C.xsph_c * w / (rhoa + n.rho);
@@ -399,11 +381,12 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
 #pragma clang fp contract(fast)
     const float rhoa = C.f_rho0 + RHO[a];
     Sum<float> sum[3];
-    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& m) {
+      const auto& n = m.r;
       if (n.pt <= 1) return;
       float dx[3];
 #pragma unroll
-      for (int d = 0; d < D; ++d) dx[d] = float(xa[d] - n.x[d]);
+      for (int d = 0; d < D; ++d) dx[d] = float(xa[d] - m.xn[d]);
       const float r = __builtin_amdgcn_sqrtf(dotf<D>(dx, dx));
       const float w = W_f(r * C.f_Hinv, C.f_wnorm);
       const float xs = C.f_xsph_c * w * __builtin_amdgcn_rcpf(rhoa + (C.f_rho0 + n.rho));
@@ -417,7 +400,7 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
   for (int d = 0; d < D; ++d) {
     const double nx = B.X[q][d][a] + out[d] * dt;
     B.X[q][d][a] = nx;
-    if constexpr (std::is_same<R, float>::value) rec_n<D>(B.PRN)[a].x[d] = nx;
+    rec_n<D, R>(B.PRN)[a].x[d] = nx;
   }
 }
 
@@ -499,12 +482,15 @@ __global__ void k_cspm_apply(SS ss) {
   const int32_t pt = B.PT[pp][a];
   if constexpr (std::is_same<R, double>::value) {
     B.RHO[q][a] = nr;
-    B.PN[a] = eos_d(nr, pt, C);
+    const double P = eos_d(nr, pt, C);
+    B.PN[a] = P;
+    rec_n<D, R>(B.PRN)[a].rho = nr;
+    rec_n<D, R>(B.PRN)[a].P = P;
   } else {
     const float d = float(nr);
     B.RHO[q][a] = d;
     B.PN[a] = eos_f(d, pt, C);
-    rec_n<D>(B.PRN)[a].rho = d;
+    rec_n<D, R>(B.PRN)[a].rho = d;
   }
 }
 

@@ -62,23 +62,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
   if constexpr (std::is_same<R, double>::value) {
     const double rhoa = sa;
     const double Pa = B.PN[a];
-    struct Nb {
-      double x[3], v[3], rho, P;
-      int32_t pt;
-    };
-    auto fetch = [&](const int32_t b) {
-      const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
-      Nb q;
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        q.x[d] = ldo(X[d], o8);
-        q.v[d] = ldo(V[d], oR);
-      }
-      q.rho = ldo(RHO, oR);
-      q.P = ldo<R>(B.PN, oR);
-      q.pt = ldo(PT, o4);
-      return q;
-    };
+    using Nb = PartnerN64<D>;
+    auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b); };
     for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
       const int32_t ptb = q.pt;
       double dx[3], dv[3], gw[3];
@@ -164,15 +149,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     const double hdt = dt * 0.5;
     double rv = rh0 + drho.get() * hdt;
     if (rv < C.rho0 && pta < 0) rv = C.rho0;
+    const double Ph = eos_d(rv, pta, C);
     B.RH[a] = rv;
-    B.PH[a] = eos_d(rv, pta, C);
+    B.PH[a] = Ph;
+    PartnerH64<D> rec{};
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       double vv = vh0[d];
       if (pta > 1) vv += av[d] * hdt;
+      const double xh = xh0[d] + vv * hdt;
       B.VH[d][a] = vv;
-      B.XH[d][a] = xh0[d] + vv * hdt;
+      B.XH[d][a] = xh;
+      rec.xn[d] = xa[d];
+      rec.xh[d] = xh;
+      rec.vh[d] = vv;
     }
+    rec.rh = rv;
+    rec.P = Ph;
+    rec.pt = pta;
+    st_rec(B.PRH, a, rec);
   } else {
     // ---------------- FP32-split
 #pragma clang fp contract(fast)
@@ -268,7 +263,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     if (rv < 0.0f && pta < 0) rv = 0.0f;
     B.RH[a] = rv;
     B.PH[a] = eos_f(rv, pta, C);
-    PartnerH<D> rec{};
+    PartnerHX<D> rec{};
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       float vv = vh0[d];
@@ -276,12 +271,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       const double xh = xh0[d] + double(vv) * (dt * 0.5);
       B.VH[d][a] = vv;
       B.XH[d][a] = xh;
-      rec.xh[d] = xh;
-      rec.vh[d] = vv;
+      rec.h.xh[d] = xh;
+      rec.h.vh[d] = vv;
+      rec.xn[d] = xa[d];
     }
-    rec.rh = rv;
-    rec.pt = pta;
-    st_rec(B.PRH, a, rec);
+    rec.h.rh = rv;
+    rec.h.pt = pta;
+    if (F.synced_prev) st_rec(B.PRH, a, rec.h);  // pass 2 derives xⁿ
+    else st_rec(B.PRH, a, rec);
   }
 }
 
@@ -320,24 +317,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
     if constexpr (std::is_same<R, double>::value) {
       const double rhoa = sha;
       Pha = B.PH[a];
-      struct Nb {
-        double xn[3], xh[3], vh[3], rh, P;
-        int32_t pt;
-      };
-      auto fetch = [&](const int32_t b) {
-        const uint32_t o8 = uint32_t(b) * 8u, oR = uint32_t(b) * uint32_t(sizeof(R)), o4 = uint32_t(b) * 4u;
-        Nb q;
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          q.xn[d] = ldo(X[d], o8);
-          q.xh[d] = ldo(XH[d], o8);
-          q.vh[d] = ldo(VH[d], oR);
-        }
-        q.rh = ldo(RH, oR);
-        q.P = ldo<R>(B.PH, oR);
-        q.pt = ldo(PT, o4);
-        return q;
-      };
+      using Nb = PartnerH64<D>;
+      auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b); };
       for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
         const int32_t ptb = q.pt;
         double dxn[3], dx[3], dv[3], gw[3];
@@ -412,11 +393,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       const double epsi = -(drho_v / rhoa) * dt;
       double rv = B.RHO[p][a] * (2 - epsi) / (2 + epsi);
       if (rv < C.rho0 && pta <= 1) rv = C.rho0;
+      const double Pn = eos_d(rv, pta, C);
       B.RHO[q][a] = rv;
-      B.PN[a] = eos_d(rv, pta, C);
+      B.PN[a] = Pn;
       if (F.corr_on) B.RH2[a] = rv;
       const double ml = pta > 1 ? 1.0 : 0.0;
       double xn[3], vn[3];
+      PartnerN64<D> rec{};
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         double vv = B.V[p][d][a];
@@ -433,11 +416,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         vn[d] = nv;
         B.V[q][d][a] = nv;
         B.X[q][d][a] = xv;
+        rec.x[d] = xv;
+        rec.v[d] = nv;
         if (F.corr_on) {
           B.VH2[d][a] = nv;
           B.XH2[d][a] = xv;
         }
       }
+      rec.rho = rv;
+      rec.P = Pn;
+      rec.pt = pta;
+      st_rec(B.PRN, a, rec);  // partner record of the next step's pass 1
       if (F.ts) {
         red_acc = sqrt(dotd<D>(av, av));
         if (F.visc_in_pass2) red_visc = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
@@ -449,15 +438,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       const float inv_rhoa = __builtin_amdgcn_rcpf(rhoa);
       const float mPa = -C.f_m0 * inv_rhoa;  // -m0/ρ_a of the pressure term
       Pha = B.PH[a];
-      using Nb = PartnerH<D>;
-      auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b); };
-      // tⁿ separation from the half-step state when x½ = xⁿ + v½·Δt/2
+      // tⁿ separation from the half-step state when x½ = xⁿ + v½·Δt/2, else
+      // from the xⁿ that pass 1 appended to the records
       const bool derive = F.synced_prev;
+      using Nb = PartnerHX<D>;
+      auto fetch = [&](const int32_t b) {
+        Nb r;
+        if (derive) {
+          r.h = ld_rec<PartnerH<D>>(B.PRH, b);
+#pragma unroll
+          for (int d = 0; d < D; ++d) r.xn[d] = 0.0;
+        } else {
+          r = ld_rec<Nb>(B.PRH, b);
+        }
+        return r;
+      };
       const double hdt = dt * 0.5;
       double ua[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) ua[d] = double(vha[d]) * hdt;
-      for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t b, const Nb& q) {
+      for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& r) {
+        const PartnerH<D>& q = r.h;
         const int32_t ptb = q.pt;
         float dxn[3], dx[3], dv[3];
 #pragma unroll
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
           const double dh = xha[d] - q.xh[d];
           dx[d] = float(dh);
           dv[d] = vha[d] - q.vh[d];
-          dxn[d] = derive ? float(dh - (ua[d] - double(q.vh[d]) * hdt)) : float(xa[d] - X[d][b]);
+          dxn[d] = derive ? float(dh - (ua[d] - double(q.vh[d]) * hdt)) : float(xa[d] - r.xn[d]);
         }
         // ∇W from pass 1 (positions xⁿ): ∇W = dxn·dwr; terms along ∇W are
         // gathered into one coefficient fg of dxn
