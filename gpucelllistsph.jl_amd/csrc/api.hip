@@ -418,6 +418,7 @@ struct sph_s {
     C.f_wnorm = float(C.wnorm);
     C.f_dwnorm = float(C.dwnorm);
     C.f_inv_rho0 = float(1.0 / C.rho0);
+    C.f_st_turn = float(C.st_c / (2.0 * M_PI * C.st_2h));
     C.f_dwr = float(-20.0 * C.dwnorm * C.Hinv);
     for (int k = 0; k < 2; ++k) {
       C.f_fb_d[k] = float(C.fb_d[k]);

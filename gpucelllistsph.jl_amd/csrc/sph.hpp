@@ -32,6 +32,7 @@ struct SphConst {
       f_dwnorm, f_fb_d[2], f_fb_r0[2], f_fb_eta2[2], f_st_2h, f_st_c, f_st_01h, f_xsph_c;
   float f_dpc_l0inv, f_dpc_eta2, f_dpc_whinv, f_dpc_lim, f_dpc_wnorm, f_dpc_lambda, f_dpc_pmin, f_dpc_pmax;
   float f_inv_rho0;  // 1/ρ0
+  float f_st_turn;   // 1.5π/(2h) in turns: cos(1.5π r/2h) = cos(2π · r·f_st_turn)
   float f_dwr;       // dW/dr / r = f_dwr (1-u)^3: -20 dwnorm H⁻¹
 };
 

@@ -112,6 +112,12 @@ __device__ __forceinline__ float hydro_drho_f(float z, float rho0) {
   return rho0 * expm1f(log1pf(z) * (1.0f / 7.0f));
 }
 
+// FP32-split ∂v∂tpF! coefficient s·cos(1.5π r/2h)/(r + 0.1h), for r < 2h: the
+// hardware cosine (argument in turns, here in [0, 0.75)) and reciprocal.
+__device__ __forceinline__ float surf_f(float r, const SphConst& C) {
+  return C.f_s * __builtin_amdgcn_cosf(r * C.f_st_turn) * __builtin_amdgcn_rcpf(r + C.f_st_01h);
+}
+
 template <typename R>
 struct Sum {  // plain sum
   R s = R(0);

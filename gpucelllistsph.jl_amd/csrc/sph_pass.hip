@@ -222,7 +222,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         }
       }
       fg *= dwr;
-      if (both && F.s_on && r < C.f_st_2h) fg += C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h) * (1.0f / C.f_m0);
+      if (both && F.s_on && r < C.f_st_2h) fg += surf_f(r, C) * (1.0f / C.f_m0);
 #pragma unroll
       for (int d = 0; d < D; ++d) ax[d] += fg * dx[d];
       if ((pta > 0) != (ptb > 0) && r < C.f_fb_r0[0]) {
@@ -497,9 +497,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
             for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
           }
           if (F.s_on) {
-            float r = sqrtf(r2);
+            const float r = __builtin_amdgcn_sqrtf(r2);
             if (r < C.f_st_2h) {
-              float scos = C.f_s * cosf(C.f_st_c * r / C.f_st_2h) / (r + C.f_st_01h);
+              const float scos = surf_f(r, C);
 #pragma unroll
               for (int d = 0; d < D; ++d) ax[d] += scos * dx[d] * (1.0f / C.f_m0);
             }
