@@ -34,8 +34,42 @@ __device__ __forceinline__ uint32_t window_bits_lds(const double2* lp, double xa
   return mk;
 }
 
-// Candidates staged in LDS per block (2D): 2048 positions, 32 KiB.
-constexpr int kNmaskLds = 2048;
+// 3D: the same over positions staged as three arrays (x, y, z).
+__device__ __forceinline__ uint32_t window_bits_lds3(const double* lx, const double* ly, const double* lz, double xa0,
+                                                     double xa1, double xa2, int64_t a, int32_t lo, int32_t hi,
+                                                     double dist2) {
+#pragma clang fp contract(off)
+  const int32_t len = hi - lo;
+  uint32_t mk = 0;
+  for (int32_t j = 0; j < len; j += 4) {
+    double c0[4], c1[4], c2[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c0[u] = lx[j + u];
+      c1[u] = ly[j + u];
+      c2[u] = lz[j + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double d0 = xa0 - c0[u];
+      const double d1 = xa1 - c1[u];
+      const double d2 = xa2 - c2[u];
+      double r2 = d0 * d0 + d1 * d1;
+      r2 = r2 + d2 * d2;
+      mk |= uint32_t(r2 < dist2) << (j + u);
+    }
+  }
+  if (len < 32) mk &= (1u << len) - 1u;
+  const int64_t self = a - lo;
+  if (self >= 0 && self < len) mk &= ~(1u << self);
+  return mk;
+}
+
+// Candidates staged in LDS per block: 2D 2036 positions, 3D 1280 per plane of
+// three stencil rows; with the padding and the row union each fits in 32 KiB,
+// which keeps five blocks per CU.
+constexpr int kNmaskLds = 2036;
+constexpr int kNmaskPlane = 1280;
 
 // Neighbour-list membership of the last rebuild, stored per particle as
 // (mask, base) entries: one 32-candidate window [base, base+32) of one stencil
@@ -46,8 +80,11 @@ constexpr int kNmaskLds = 2048;
 //
 // 2D: the block stages the positions of its row union (row_union) in LDS once
 // (coalesced) and every lane tests its candidates there, instead of each lane
-// fetching each of its ~100 candidates through the vector L1.  A block whose
-// union exceeds the LDS budget reads the candidates from global memory.
+// fetching each of its ~100 candidates through the vector L1.  3D: the same
+// plane by plane (three stencil rows at a time, ~1000 candidates per particle
+// in all).  A block whose union exceeds the LDS budget reads the candidates
+// from global memory (per plane in 3D).  The tests are the same expression in
+// every path, so the bits are the same.
 template <int D, typename R, typename SS>
 __global__ __launch_bounds__(256) void k_nmask(SS ss) {
   GCLSPH_SCENE(ss);
@@ -95,6 +132,57 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
       B.NC[a] = cnt <= cap ? cnt : -1;
       return;
     }
+  } else {
+    __shared__ double lx[kNmaskPlane + 4], ly[kNmaskPlane + 4], lz[kNmaskPlane + 4];  // +4 as in 2D
+    __shared__ RowUnion u;
+    double xa0 = 0.0, xa1 = 0.0, xa2 = 0.0;
+    Rows<D> rows{};
+    if (active) {
+      xa0 = XM[0][a];
+      xa1 = XM[1][a];
+      xa2 = XM[2][a];
+      rows = stencil_rows<D>(G, B.start, B.skey[a]);
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int pl = 0; pl < 3; ++pl) {  // rows 3·pl .. 3·pl+2 lie in the plane dz = pl - 1
+      if (threadIdx.x == 0) row_union(u, B.start, B.skey, G, B.n, a0, pl - 1);
+      __syncthreads();
+      const int32_t total = u.uo[3];
+      const bool staged = total <= kNmaskPlane;  // uniform over the block
+      if (staged) {
+        for (int32_t i = threadIdx.x; i < total; i += kBlock) {
+          const int k = (i >= u.uo[1]) + (i >= u.uo[2]);
+          const int32_t b = u.ub[k] + (i - u.uo[k]);
+          lx[i] = XM[0][b];
+          ly[i] = XM[1][b];
+          lz[i] = XM[2][b];
+        }
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+          const int k = 3 * pl + kk;
+          const int32_t off = u.uo[kk] - u.ub[kk];  // candidate b at l?[b + off]
+          const int32_t rb = rows.b[k], re = rows.e[k];
+          for (int32_t w0 = rb; w0 < re; w0 += 32) {
+            const int32_t hi = re < w0 + 32 ? re : w0 + 32;
+            const uint32_t mk =
+                staged ? window_bits_lds3(lx + (w0 + off), ly + (w0 + off), lz + (w0 + off), xa0, xa1, xa2, a, w0, hi,
+                                          dist2)
+                       : window_bits<D>(XM[0], XM[1], XM[2], xa0, xa1, xa2, a, w0, hi, dist2);
+            if (mk != 0) {
+              if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
+              ++cnt;
+            }
+          }
+        }
+      }
+      __syncthreads();  // the next plane overwrites the staged positions
+    }
+    if (active) B.NC[a] = cnt <= cap ? cnt : -1;
+    return;
   }
   if (!active) return;
   const double xa0 = XM[0][a], xa1 = XM[1][a], xa2 = D == 3 ? XM[2][a] : 0.0;
