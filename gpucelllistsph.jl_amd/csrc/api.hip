@@ -571,7 +571,7 @@ struct sph_s {
       gather<double>(t, a, sb.perm, n, stream);
     };
     auto permute_r = [&](R* a) {
-      R* t = B.DV[0];
+      R* t = reinterpret_cast<R*>(B.TMP);  // n doubles hold n elements of either precision
       GCL_HIP(hipMemcpyAsync(t, a, sizeof(R) * size_t(n), hipMemcpyDeviceToDevice, stream));
       gather<R>(t, a, sb.perm, n, stream);
     };
@@ -579,6 +579,7 @@ struct sph_s {
       permute_d(B.CDX[k]);
       permute_d(B.SUMGW[k]);
       permute_r(B.ACC[k]);
+      permute_r(B.DV[k]);
     }
     permute_d(B.SUMW);
     permute_r(B.DRHO);
@@ -1458,6 +1459,15 @@ int sph_get_field(sph_t h, int field, double* out) {
         break;
       case SPH_FIELD_PTYPE:
         put(unsort<int32_t>(h->ptcur(), id, n, h->stream), 0);
+        break;
+      case SPH_FIELD_DPC:  // Δv_dpc of the last step's dpcreg!; zero when the last call ran without DPC
+        if (!h->p_from_dpc) {
+          std::memset(out, 0, sizeof(double) * size_t(n) * size_t(h->dim));
+        } else if (sp) {
+          vec_r(h->Bf.DV, 0.0);
+        } else {
+          vec_r(h->Bd.DV, 0.0);
+        }
         break;
       default:
         throw Error(kErrArgument, "unknown field");

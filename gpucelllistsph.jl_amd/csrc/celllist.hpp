@@ -136,14 +136,6 @@ __device__ __forceinline__ Rows<D> stencil_rows(const CellGeom& g, const int32_t
   return r;
 }
 
-template <int N, typename T>
-__device__ __forceinline__ T select_k(const T (&v)[N], int k) {
-  T r = v[0];
-#pragma unroll
-  for (int i = 1; i < N; ++i) r = k == i ? v[i] : r;
-  return r;
-}
-
 // Bins positions `x` (D pointers, input order, element type double or float)
 // into cells, and writes the sorted order (perm, skey, start).  The order
 // inside a cell is by input index, so the result is deterministic.

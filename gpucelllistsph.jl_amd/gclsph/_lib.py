@@ -24,7 +24,7 @@ PARAMS = ("dx", "h", "H", "rho0", "m0", "dt", "alpha", "c0", "gamma", "delta", "
 PARAM_INDEX = {k: i for i, k in enumerate(PARAMS)}
 
 FIELD_X, FIELD_V, FIELD_RHO, FIELD_P, FIELD_ACC, FIELD_SUMW, FIELD_SUMGRADW = 0, 1, 2, 3, 4, 5, 6
-FIELD_DRHO, FIELD_PTYPE = 10, 12
+FIELD_DRHO, FIELD_PTYPE, FIELD_DPC = 10, 12, 13
 
 
 class GCLInfo(C.Structure):

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import os
 import warnings
 import weakref
 from typing import Optional, Sequence, Tuple
@@ -369,12 +370,17 @@ def batch_stepsolve_(batch: SPHBatch, n: int = 1, timestepping: bool = False,
 def timesolve_(prob: SPHProblem, batch: int = 10, timeframe: float = 1.0, writetime: float = 0, path=None,
                pvc: bool = False, vtkvars=("Acceleration", "Velocity", "Pressure"), timestepping: bool = False,
                timelims: Optional[Tuple[float, float]] = None, anim: bool = False, verbose: bool = True):
-    """timesolve!(prob; batch, timeframe, …) — src/sphproblem.jl:420-511 (the batch loop).
-
-    VTK / animation output (the `path`, `pvc`, `anim` branches) is not part of
-    this library; asking for it raises NotImplementedError."""
-    if (writetime > 0 and path is not None) or anim:
-        raise NotImplementedError("VTK / animation output is not implemented; run without path/anim")
+    """timesolve!(prob; batch, timeframe, writetime, path, pvc, vtkvars, …) —
+    src/sphproblem.jl:420-511.  Steps in batches of `batch` until the simulated
+    time passes `timeframe`.  With writetime > 0 and a path, writes
+    OUTPUT_<nnnnn>.vtp files (nnnnn = the batch counter, as the reference names
+    them: the first write after the start reuses OUTPUT_00000) whenever the
+    time passes the next multiple of writetime, and with pvc a ParaView
+    collection OUTPUT_PVC.pvd.  The animation branch (Plots.jl) is not part of
+    this library: anim=True raises NotImplementedError."""
+    from .io import paraview_collection, vtk_save, writevtk
+    if anim:
+        raise NotImplementedError("animation output is not implemented; write VTK files (path, pvc) instead")
     lim = prob.CFL * prob.H / (3 * prob.c0)
     if timelims is None:
         timelims = (math.sqrt(_EPS), lim)
@@ -389,11 +395,25 @@ def timesolve_(prob: SPHProblem, batch: int = 10, timeframe: float = 1.0, writet
             print(f"Δt limitss: {timelims}")
         print(f"Batch: {batch}")
         print(f"NUI: {prob.nui}")
-    diag = None
+    nt = prob.etime + writetime
     i = 0
+    pvd = None
+    write = writetime > 0 and path is not None
+    if write:
+        if pvc:
+            pvd = paraview_collection(os.path.join(path, "OUTPUT_PVC"))
+        writevtk(prob, os.path.join(path, f"OUTPUT_{i:05d}"), vtkvars, get_points(prob), pvd=pvd, writetime=True)
+    diag = None
     while prob.etime <= timeframe:
         diag = stepsolve_(prob, batch, timestepping=timestepping, timelims=timelims)
+        if writetime > 0 and nt < prob.etime:
+            nt += writetime
+            if path is not None:
+                writevtk(prob, os.path.join(path, f"OUTPUT_{i:05d}"), vtkvars, get_points(prob), pvd=pvd,
+                         writetime=True)
         i += 1
+    if write and pvc:
+        vtk_save(pvd)
     return diag
 
 
@@ -427,6 +447,14 @@ def get_sumgradw(prob: SPHProblem):
 
 def get_ptype(prob: SPHProblem):
     return prob.field(_lib.FIELD_PTYPE, False).astype(np.int32)
+
+
+def get_dpccorr(prob: SPHProblem):
+    """get_dpccorr (src/sphproblem.jl:368-370).  The reference's getter reads a
+    field its SPHProblem does not define; this returns what the name says: the
+    per-particle velocity correction Δv_dpc of the last step's dpcreg!
+    (src/gpukernels_sph.jl:1500-1553), zero when the last call ran without DPC."""
+    return prob.field(_lib.FIELD_DPC, True)
 
 
 def get_simtime(prob: SPHProblem) -> float:

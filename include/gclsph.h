@@ -135,7 +135,9 @@ enum {
   SPH_FIELD_SUMW = 5,  /* get_sumw        :380 */
   SPH_FIELD_SUMGRADW = 6, /* get_sumgradw :384 */
   SPH_FIELD_DRHO = 10, /* prob.∑∂ρ∂t of the last step */
-  SPH_FIELD_PTYPE = 12 /* get_ptype :388 (as doubles) */
+  SPH_FIELD_PTYPE = 12, /* get_ptype :388 (as doubles) */
+  SPH_FIELD_DPC = 13   /* get_dpccorr :368-370, which names a field the reference's SPHProblem lacks:
+                          here the per-particle Δv_dpc of the last step's dpcreg! (zero without DPC) */
 };
 int sph_get_field(sph_t h, int field, double* out);
 

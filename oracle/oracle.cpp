@@ -408,6 +408,7 @@ struct Sph {
   std::vector<double> rho, rhoh, P, buf, drho, sumW, W;
   std::vector<int32_t> ptype;
   std::vector<double> v[3], vh[3], xh[3], acc[3], buf2[3], cdx[3], sumgW[3], gW[3];
+  std::vector<double> dvdpc[3];  // Δv_dpc of the last dpcreg! (buf2 is reused by XSPH)
   // pairs view: first pairsn entries (or whole buffer before first update)
   const int32_t* pr = nullptr;
   int64_t npr = 0;
@@ -756,6 +757,7 @@ void dpc(Sph& S, double dpckernlim) {
       S.buf2[k][j] -= dd[k];
     }
   }
+  for (int k = 0; k < D; ++k) S.dvdpc[k] = S.buf2[k];
   for (int64_t i = 0; i < S.N; ++i)
     if (S.ptype[i] > 1)
       for (int k = 0; k < D; ++k) {
@@ -1094,6 +1096,7 @@ orc_sph* orc_sph_create(orc_gcl* sys, const double* par /* Params order */, cons
     S.xh[k] = sys->d.x[k];
     S.acc[k].assign(N, 0.0);
     S.buf2[k].assign(N, 0.0);
+    S.dvdpc[k].assign(N, 0.0);
     S.cdx[k].assign(N, 0.0);
     S.sumgW[k].assign(N, 0.0);
   }
@@ -1127,6 +1130,7 @@ int orc_sph_get_field(orc_sph* h, int field, double* out) {
     case 9: vec(S.xh); break;
     case 10: std::copy(S.drho.begin(), S.drho.end(), out); break;
     case 11: vec(S.cdx); break;
+    case 13: vec(S.dvdpc); break;
     default: return fail("unknown field");
   }
   return 0;

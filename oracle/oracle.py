@@ -167,7 +167,7 @@ class CellList:
 
 FIELDS = {"x": (0, True), "v": (1, True), "rho": (2, False), "P": (3, False), "acc": (4, True), "sumW": (5, False),
           "sumgradW": (6, True), "rhoh": (7, False), "vh": (8, True), "xh": (9, True), "drho": (10, False),
-          "cdx": (11, True)}
+          "cdx": (11, True), "dpc": (13, True)}
 
 
 class SPHProblem:
