@@ -735,11 +735,14 @@ void run_steps(sph_s* const* P, unsigned count, LaunchSet& L, hipStream_t st, in
   // timing, else the fastest measured on the dam-break scenes per dimension
   // and precision until it has been tuned.
   int pv[2];
-  if (D == 2) {
-    pv[0] = pv[1] = std::is_same<R, float>::value ? 3 : 2;
+  if (std::is_same<R, float>::value) {
+    pv[0] = 2;  // prefetch, >= 3 waves per SIMD
+    pv[1] = 0;  // prefetch, compiler's occupancy
+  } else if (D == 2) {
+    pv[0] = pv[1] = 2;
   } else {
-    pv[0] = std::is_same<R, float>::value ? 0 : 1;
-    pv[1] = 0;
+    pv[0] = 0;
+    pv[1] = 1;
   }
   bool forced = false;
   if (const char* env = std::getenv("GCLSPH_PASS_VARIANT")) {
