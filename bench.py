@@ -191,7 +191,11 @@ def main():
             "rebuilds_in_timed_steps": res32["updaten"],
             "precision": "FP32-split",
         },
-        "stage_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in stages.items() if v[1] > 0},
+        # ms per timed step (a stage that does not run every step, such as CSPM
+        # every cspmn steps under "corrections", is spread over all steps) and
+        # how many of the timed steps ran it
+        "stage_ms_per_step": {k: v[0] / args.steps for k, v in stages.items() if v[1] > 0},
+        "stage_launches": {k: v[1] for k, v in stages.items() if v[1] > 0},
         "roofline": {
             "bound": "hbm",
             "kernel": f"k_{dom}",
@@ -209,7 +213,7 @@ def main():
     if res64 is not None:
         v64 = n * args.steps / res64["seconds"]
         out["f64"] = {"value": v64, "ms_per_step": res64["seconds"] / args.steps * 1e3,
-                      "stage_ms_per_step": {k: v[0] / max(v[1], 1) for k, v in res64["timing"].items() if v[1] > 0}}
+                      "stage_ms_per_step": {k: v[0] / args.steps for k, v in res64["timing"].items() if v[1] > 0}}
         out["f32split_over_f64"] = value / v64
     if not args.no_ensemble:
         out["ensemble"] = run_ensemble(args.steps, min(args.warmup, 3))

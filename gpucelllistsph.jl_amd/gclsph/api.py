@@ -376,9 +376,12 @@ def timesolve_(prob: SPHProblem, batch: int = 10, timeframe: float = 1.0, writet
     OUTPUT_<nnnnn>.vtp files (nnnnn = the batch counter, as the reference names
     them: the first write after the start reuses OUTPUT_00000) whenever the
     time passes the next multiple of writetime, and with pvc a ParaView
-    collection OUTPUT_PVC.pvd.  The animation branch (Plots.jl) is not part of
-    this library: anim=True raises NotImplementedError."""
-    from .io import paraview_collection, vtk_save, writevtk
+    collection OUTPUT_PVC.pvd.  Each file is compressed and written on a
+    background thread while the next batch steps (the fields are copied to
+    the host first); all files are complete when the call returns.  The
+    animation branch (Plots.jl) is not part of this library: anim=True raises
+    NotImplementedError."""
+    from .io import VtkWriter, paraview_collection, vtk_save
     if anim:
         raise NotImplementedError("animation output is not implemented; write VTK files (path, pvc) instead")
     lim = prob.CFL * prob.H / (3 * prob.c0)
@@ -399,19 +402,23 @@ def timesolve_(prob: SPHProblem, batch: int = 10, timeframe: float = 1.0, writet
     i = 0
     pvd = None
     write = writetime > 0 and path is not None
-    if write:
-        if pvc:
-            pvd = paraview_collection(os.path.join(path, "OUTPUT_PVC"))
-        writevtk(prob, os.path.join(path, f"OUTPUT_{i:05d}"), vtkvars, get_points(prob), pvd=pvd, writetime=True)
-    diag = None
-    while prob.etime <= timeframe:
-        diag = stepsolve_(prob, batch, timestepping=timestepping, timelims=timelims)
-        if writetime > 0 and nt < prob.etime:
-            nt += writetime
-            if path is not None:
-                writevtk(prob, os.path.join(path, f"OUTPUT_{i:05d}"), vtkvars, get_points(prob), pvd=pvd,
-                         writetime=True)
-        i += 1
+    writer = VtkWriter() if write else None
+    try:
+        if write:
+            if pvc:
+                pvd = paraview_collection(os.path.join(path, "OUTPUT_PVC"))
+            writer.submit(prob, os.path.join(path, f"OUTPUT_{i:05d}"), vtkvars, pvd=pvd, writetime=True)
+        diag = None
+        while prob.etime <= timeframe:
+            diag = stepsolve_(prob, batch, timestepping=timestepping, timelims=timelims)
+            if writetime > 0 and nt < prob.etime:
+                nt += writetime
+                if path is not None:
+                    writer.submit(prob, os.path.join(path, f"OUTPUT_{i:05d}"), vtkvars, pvd=pvd, writetime=True)
+            i += 1
+    finally:
+        if writer is not None:
+            writer.close()
     if write and pvc:
         vtk_save(pvd)
     return diag

@@ -138,11 +138,10 @@ def create_vtp_file(filename: str, x: Sequence[np.ndarray], expdict: Dict[str, o
     return path
 
 
-def writevtk(prob, filename: str, vtkvars: Iterable[str], cpupoints=None, pvd=None, writetime: bool = False) -> str:
-    """writevtk(prob, filename, vtkvars, cpupoints; pvd, writetime) —
-    src/writevtk.jl:29-43.  vtkvars ⊂ ("Density", "Pressure", "Type",
-    "Acceleration", "Velocity", "∑W", "∑∇W", "DPC"); with writetime the file
-    joins `pvd` at the problem's simulated time."""
+def collect_vtk(prob, vtkvars: Iterable[str], cpupoints=None):
+    """The host copies writevtk needs: (points, {variable: array or tuple of
+    arrays}, simulated time).  Only this step touches the device; the file can
+    then be written from another thread while the problem steps on."""
     g = _getters()
     if cpupoints is None:
         cpupoints = g["points"](prob)
@@ -151,7 +150,41 @@ def writevtk(prob, filename: str, vtkvars: Iterable[str], cpupoints=None, pvd=No
         if name in vtkvars:
             val = g[name](prob)
             expdict[name] = tuple(val) if isinstance(val, tuple) else val
-    return create_vtp_file(filename, cpupoints, expdict, pvd=pvd, time=prob.etime if writetime else None)
+    return cpupoints, expdict, prob.etime
+
+
+def writevtk(prob, filename: str, vtkvars: Iterable[str], cpupoints=None, pvd=None, writetime: bool = False) -> str:
+    """writevtk(prob, filename, vtkvars, cpupoints; pvd, writetime) —
+    src/writevtk.jl:29-43.  vtkvars ⊂ ("Density", "Pressure", "Type",
+    "Acceleration", "Velocity", "∑W", "∑∇W", "DPC"); with writetime the file
+    joins `pvd` at the problem's simulated time."""
+    pts, expdict, etime = collect_vtk(prob, vtkvars, cpupoints)
+    return create_vtp_file(filename, pts, expdict, pvd=pvd, time=etime if writetime else None)
+
+
+class VtkWriter:
+    """Writes .vtp files on one background thread, in submission order, so
+    that compression and disk output overlap the next batch of steps (the
+    device copies are taken on the caller's thread by collect_vtk; zlib and
+    file output release the interpreter lock).  close() waits for every file
+    and re-raises the first error."""
+
+    def __init__(self):
+        from concurrent.futures import ThreadPoolExecutor
+        self._pool = ThreadPoolExecutor(max_workers=1, thread_name_prefix="vtk-writer")
+        self._futures = []
+
+    def submit(self, prob, filename: str, vtkvars: Iterable[str], pvd=None, writetime: bool = False):
+        pts, expdict, etime = collect_vtk(prob, vtkvars)
+        self._futures.append(self._pool.submit(create_vtp_file, filename, pts, expdict, pvd,
+                                               etime if writetime else None))
+
+    def close(self) -> List[str]:
+        try:
+            return [f.result() for f in self._futures]
+        finally:
+            self._pool.shutdown(wait=True)
+            self._futures = []
 
 
 class paraview_collection:  # noqa: N801  (the reference's name, WriteVTK.paraview_collection)

@@ -118,3 +118,32 @@ def test_vtk_binary_block_header():
     data = base64.b64decode(txt[44:])
     assert len(data) == head[3]
     np.testing.assert_array_equal(np.frombuffer(zlib.decompress(data), dtype="<f8"), a)
+
+
+def test_background_writer_order_and_content(tmp_path):
+    """VtkWriter: files written off-thread, in submission order, registered in
+    the collection in that order, with the values taken at submission time."""
+    prob = FakeProblem(2, 300)
+    pvd = gio.paraview_collection(str(tmp_path / "OUT"))
+    w = gio.VtkWriter()
+    expected = []
+    for k in range(4):
+        prob.etime = 0.1 * (k + 1)
+        prob.f["rho"] = prob.f["rho"] + 1.0  # a new array each time, as the getters return
+        expected.append(prob.f["rho"].copy())
+        w.submit(prob, str(tmp_path / f"OUTPUT_{k:05d}"), ("Density", "Velocity"), pvd=pvd, writetime=True)
+    paths = w.close()
+    assert paths == [str(tmp_path / f"OUTPUT_{k:05d}.vtp") for k in range(4)]
+    assert [t for t, _ in pvd.entries] == [0.1 * (k + 1) for k in range(4)]
+    for k, p in enumerate(paths):
+        d = gio.read_vtp(p)
+        np.testing.assert_array_equal(d["Density"], expected[k])
+        np.testing.assert_array_equal(d["Velocity"][:, 1], prob.f["v"][1])
+
+
+def test_background_writer_reraises(tmp_path):
+    prob = FakeProblem(2, 10)
+    w = gio.VtkWriter()
+    w.submit(prob, str(tmp_path / "missing_dir" / "OUTPUT_00000"), ("Density",))
+    with pytest.raises(OSError):
+        w.close()
