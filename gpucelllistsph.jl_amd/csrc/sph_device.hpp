@@ -95,6 +95,22 @@ __device__ __forceinline__ double eos_d(double rho, int32_t pt, const SphConst& 
   if (pt <= 1 && rho < C.rho0) return 0.0;
   return C.P0 * (powi_d(rho / C.rho0, C.gamma) - 1);
 }
+// ptype(a)·ptype(b) > 0 of the reference (an Int64 product) for int32 types:
+// both positive or both negative, without the 64-bit multiply.
+__device__ __forceinline__ bool same_kind(int32_t pa, int32_t pb) { return (pa > 0 && pb > 0) || (pa < 0 && pb < 0); }
+
+// FP32-split 1/ρb and 1/(ρa + ρb) from one hardware reciprocal of
+// ρb(ρa + ρb) (densities near ρ0, so the product stays far from the FP32
+// range limits).
+struct RhoInv {
+  float inv_rhob, inv_sum;
+};
+__device__ __forceinline__ RhoInv rho_inv2(float rhoa, float rhob) {
+  const float sum = rhoa + rhob;
+  const float R = __builtin_amdgcn_rcpf(rhob * sum);
+  return {sum * R, rhob * R};
+}
+
 // FP32-split EOS on δρ: P0((1+e)^7 - 1) = P0 e (7 + 21e + 35e² + 35e³ + 21e⁴ + 7e⁵ + e⁶).
 __device__ __forceinline__ float eos_f(float drho, int32_t pt, const SphConst& C) {
   if (pt <= 1 && drho < 0.0f) return 0.0f;

@@ -91,7 +91,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         }
         drho.add(s);
       }
-      if ((long long)pta * (long long)ptb > 0) {  // ∂v∂t! (:755-772)
+      if (same_kind(pta, ptb)) {  // ∂v∂t! (:755-772)
         double Pfac = (Pa + q.P) / (rhoa * rhob);
 #pragma unroll
         for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
@@ -195,7 +195,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       const float dvdx = dotf<D>(dv, dx);
       const float drb = q.rho;
       const float rhob = C.f_rho0 + drb;
-      const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
+      const RhoInv ri = rho_inv2(rhoa, rhob);
+      const float inv_rhob = ri.inv_rhob;
       // The common terms are evaluated for every pair and selected (no
       // divergent branches in the hot loop): continuity + δ-SPH, pressure, AV.
       const bool both = pta > 0 && ptb > 0;
@@ -208,15 +209,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         drho.add(both ? (pta > 1 ? sd : s) : 0.0f);
       }
       const float Pt = (Pa + eos_f(drb, ptb, C)) * inv_rhob * mPa;
-      float fg = (long long)pta * (long long)ptb > 0 ? Pt : 0.0f;
+      float fg = same_kind(pta, ptb) ? Pt : 0.0f;
       {
         const float dmu = C.f_h * fminf(dvdx, 0.0f) * inv_den;  // AV acts on approaching pairs only
-        const float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
+        const float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * ri.inv_sum);
         fg -= both ? dPi * C.f_m0 : 0.0f;
       }
       if (both) {
         if (F.nu_on) {
-          float term = 4.0f * C.f_nu * C.f_m0 * (r2 * dwr) * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
+          float term = 4.0f * C.f_nu * C.f_m0 * (r2 * dwr) * (ri.inv_sum * inv_den);
 #pragma unroll
           for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
         }
@@ -348,7 +349,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
           }
           drho.add(s);
         }
-        if ((long long)pta * (long long)ptb > 0) {
+        if (same_kind(pta, ptb)) {
           double Pfac = (Pha + q.P) / (rhoa * rhob);
 #pragma unroll
           for (int d = 0; d < D; ++d) ax[d] += -C.m0 * Pfac * gw[d];
@@ -478,7 +479,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         const float dvdx = dotf<D>(dv, dx);
         const float drb = q.rh;
         const float rhob = C.f_rho0 + drb;
-        const float inv_rhob = __builtin_amdgcn_rcpf(rhob);
+        const RhoInv ri = rho_inv2(rhoa, rhob);
+        const float inv_rhob = ri.inv_rhob;
         // common terms evaluated for every pair and selected, as in pass 1
         const bool both = pta > 0 && ptb > 0;
         float ax[3] = {0.0f, 0.0f, 0.0f};
@@ -492,7 +494,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         }
         if (both) {
           if (F.nu_on) {
-            float term = 4.0f * C.f_nu * C.f_m0 * dxg * (__builtin_amdgcn_rcpf(rhoa + rhob) * inv_den);
+            float term = 4.0f * C.f_nu * C.f_m0 * dxg * (ri.inv_sum * inv_den);
 #pragma unroll
             for (int d = 0; d < D; ++d) ax[d] += term * dv[d];
           }
@@ -506,10 +508,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
           }
         }
         const float Pt = (Pha + eos_f(drb, ptb, C)) * inv_rhob * mPa;
-        float fg = (long long)pta * (long long)ptb > 0 ? Pt : 0.0f;
+        float fg = same_kind(pta, ptb) ? Pt : 0.0f;
         {
           const float dmu = C.f_h * fminf(dvdx, 0.0f) * inv_den;
-          const float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * __builtin_amdgcn_rcpf(rhoa + rhob));
+          const float dPi = (-C.f_alpha * C.f_c0 * dmu) * (2.0f * ri.inv_sum);
           fg -= both ? dPi * C.f_m0 : 0.0f;
         }
         fg *= dwr;
