@@ -9,29 +9,59 @@
 
 namespace gclsph {
 
+// acc·16 + the four bits (r_u < dist2), candidate 0 the highest: four FP64
+// compares into SGPR lane masks, then four add-with-carry steps acc+acc+bit.
+// Each mask is read three instructions after it is written (gfx950 needs two
+// wait states between a VALU write of an SGPR and its use as a carry-in).
+__device__ __forceinline__ uint32_t push_lt4(uint32_t acc, double r0, double r1, double r2, double r3,
+                                             double dist2) {
+  uint64_t c0, c1, c2, c3, co;
+  asm("v_cmp_gt_f64_e64 %1, %6, %7\n\t"
+      "v_cmp_gt_f64_e64 %2, %6, %8\n\t"
+      "v_cmp_gt_f64_e64 %3, %6, %9\n\t"
+      "v_cmp_gt_f64_e64 %4, %6, %10\n\t"
+      "v_addc_co_u32_e64 %0, %5, %0, %0, %1\n\t"
+      "v_addc_co_u32_e64 %0, %5, %0, %0, %2\n\t"
+      "v_addc_co_u32_e64 %0, %5, %0, %0, %3\n\t"
+      "v_addc_co_u32_e64 %0, %5, %0, %0, %4"
+      : "+v"(acc), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(c3), "=&s"(co)
+      : "s"(dist2), "v"(r0), "v"(r1), "v"(r2), "v"(r3));
+  return acc;
+}
+
+// Window mask from the pushed bits: len candidates pushed in groups of four
+// (candidate t at bit nc-1-t, nc = len rounded up to 4) become bit t, the
+// candidates past len and the particle itself cleared.
+__device__ __forceinline__ uint32_t window_mask(uint32_t acc, int32_t len, int64_t a, int32_t lo) {
+  const int32_t nc = (len + 3) & ~3;
+  uint32_t mk = __builtin_bitreverse32(acc << (32 - nc));
+  if (len < 32) mk &= (1u << len) - 1u;
+  const int64_t self = a - lo;
+  if (self >= 0 && self < len) mk &= ~(1u << self);
+  return mk;
+}
+
 // window_bits over candidate positions staged in LDS (2D: (x, y) pairs): lp
 // points at candidate lo.  Same arithmetic, so the same bits.
 __device__ __forceinline__ uint32_t window_bits_lds(const double2* lp, double xa0, double xa1, int64_t a,
                                                     int32_t lo, int32_t hi, double dist2) {
 #pragma clang fp contract(off)
   const int32_t len = hi - lo;
-  uint32_t mk = 0;
+  uint32_t acc = 0;
   for (int32_t j = 0; j < len; j += 4) {
     double2 c[4];
+    double r[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) c[u] = lp[j + u];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const double d0 = xa0 - c[u].x;
       const double d1 = xa1 - c[u].y;
-      const double r2 = d0 * d0 + d1 * d1;
-      mk |= uint32_t(r2 < dist2) << (j + u);
+      r[u] = d0 * d0 + d1 * d1;
     }
+    acc = push_lt4(acc, r[0], r[1], r[2], r[3], dist2);
   }
-  if (len < 32) mk &= (1u << len) - 1u;
-  const int64_t self = a - lo;
-  if (self >= 0 && self < len) mk &= ~(1u << self);
-  return mk;
+  return window_mask(acc, len, a, lo);
 }
 
 // 3D: the same over positions staged as three arrays (x, y, z).
@@ -40,9 +70,10 @@ __device__ __forceinline__ uint32_t window_bits_lds3(const double* lx, const dou
                                                      double dist2) {
 #pragma clang fp contract(off)
   const int32_t len = hi - lo;
-  uint32_t mk = 0;
+  uint32_t acc = 0;
   for (int32_t j = 0; j < len; j += 4) {
     double c0[4], c1[4], c2[4];
+    double r[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       c0[u] = lx[j + u];
@@ -55,14 +86,11 @@ __device__ __forceinline__ uint32_t window_bits_lds3(const double* lx, const dou
       const double d1 = xa1 - c1[u];
       const double d2 = xa2 - c2[u];
       double r2 = d0 * d0 + d1 * d1;
-      r2 = r2 + d2 * d2;
-      mk |= uint32_t(r2 < dist2) << (j + u);
+      r[u] = r2 + d2 * d2;
     }
+    acc = push_lt4(acc, r[0], r[1], r[2], r[3], dist2);
   }
-  if (len < 32) mk &= (1u << len) - 1u;
-  const int64_t self = a - lo;
-  if (self >= 0 && self < len) mk &= ~(1u << self);
-  return mk;
+  return window_mask(acc, len, a, lo);
 }
 
 // Candidates staged in LDS per block: 2D 2036 positions, 3D 1280 per plane of

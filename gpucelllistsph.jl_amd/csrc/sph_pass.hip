@@ -454,10 +454,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         }
         return r;
       };
-      const double hdt = dt * 0.5;
-      double ua[3];
-#pragma unroll
-      for (int d = 0; d < D; ++d) ua[d] = double(vha[d]) * hdt;
+      // derived: xⁿa - xⁿb = (x½a - x½b) - (v½a - v½b)·Δt/2, the velocity part
+      // in FP32 (its size is |Δv|·Δt/2, far below the separation's FP32 ulp)
+      const float fhdt = float(dt * 0.5);
       for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& r) {
         const PartnerH<D>& q = r.h;
         const int32_t ptb = q.pt;
@@ -467,7 +466,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
           const double dh = xha[d] - q.xh[d];
           dx[d] = float(dh);
           dv[d] = vha[d] - q.vh[d];
-          dxn[d] = derive ? float(dh - (ua[d] - double(q.vh[d]) * hdt)) : float(xa[d] - r.xn[d]);
+          dxn[d] = derive ? dx[d] - dv[d] * fhdt : float(xa[d] - r.xn[d]);
         }
         // ∇W from pass 1 (positions xⁿ): ∇W = dxn·dwr; terms along ∇W are
         // gathered into one coefficient fg of dxn
