@@ -114,8 +114,10 @@ struct SphBufs {
 // also needs the partner's tⁿ position (∇W at tⁿ): when the half step started
 // from xⁿ (no correction in the previous step) x½ = xⁿ + v½·Δt/2, so the tⁿ
 // separation is the x½ separation minus (v½_a - v½_b)·Δt/2, exact to FP64
-// rounding; otherwise the records are PartnerHX, which append xⁿ (the step
-// flag synced_prev tells both passes which layout the step uses).  The
+// rounding; otherwise the records are PartnerHX, which append x½ - xⁿ in FP32
+// (a displacement of order v·Δt, so its FP32 rounding lies far below the FP32
+// rounding of the separation itself; the step flag synced_prev tells both
+// passes which layout the step uses).  The
 // pressure is not stored: it is the EOS of (rho, pt), recomputed with the same
 // function that fills PN / PH.
 template <int D>
@@ -136,10 +138,10 @@ static_assert(sizeof(PartnerN<2>) == 32 && sizeof(PartnerN<3>) == 48, "partner r
 template <int D>
 struct alignas(16) PartnerHX {
   PartnerH<D> h;
-  double xn[D];
+  float dn[D];  // x½ - xⁿ
 };
 static_assert(sizeof(PartnerH<2>) == 32 && sizeof(PartnerH<3>) == 48, "partner record sizes");
-static_assert(sizeof(PartnerHX<2>) == 48 && sizeof(PartnerHX<3>) == 80, "partner record sizes");
+static_assert(sizeof(PartnerHX<2>) == 48 && sizeof(PartnerHX<3>) == 64, "partner record sizes");
 
 // FP64 path: the pressure is stored (its EOS costs an FP64 division and a
 // power per partner), and pass 2's record always carries xⁿ (an FP64-exact

@@ -463,14 +463,21 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
     va[d] = V[d][a];
   }
   // v, ρ and ptype of a partner from its record (updated by the DPC), xⁿ from
-  // the position arrays
+  // the position arrays.  FP32-split: only the record words past the updated
+  // position (which this pass does not use) are loaded.
   struct Nb {
     typename Records<D, R>::N r;
     double xn[3];
   };
+  using Rn = typename Records<D, R>::N;
   auto fetch = [&](const int32_t b) {
     Nb n;
-    n.r = ld_rec<typename Records<D, R>::N>(B.PRN, b);
+    if constexpr (std::is_same<R, float>::value) {
+      static_assert(offsetof(Rn, v) >= 16, "v starts in the second record word");
+      n.r = ld_rec_tail<Rn, 1>(B.PRN, b);
+    } else {
+      n.r = ld_rec<Rn>(B.PRN, b);
+    }
 #pragma unroll
     for (int d = 0; d < D; ++d) n.xn[d] = XN[d][b];
     return n;

@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       B.XH[d][a] = xh;
       rec.h.xh[d] = xh;
       rec.h.vh[d] = vv;
-      rec.xn[d] = xa[d];
+      rec.dn[d] = float(xh - xa[d]);
     }
     rec.h.rh = rv;
     rec.h.pt = pta;
@@ -448,15 +448,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
         if (derive) {
           r.h = ld_rec<PartnerH<D>>(B.PRH, b);
 #pragma unroll
-          for (int d = 0; d < D; ++d) r.xn[d] = 0.0;
+          for (int d = 0; d < D; ++d) r.dn[d] = 0.0f;
         } else {
           r = ld_rec<Nb>(B.PRH, b);
         }
         return r;
       };
-      // derived: xⁿa - xⁿb = (x½a - x½b) - (v½a - v½b)·Δt/2, the velocity part
-      // in FP32 (its size is |Δv|·Δt/2, far below the separation's FP32 ulp)
+      // xⁿa - xⁿb = (x½a - x½b) - (da - db) with d = x½ - xⁿ, the displacement
+      // part in FP32 (its size is of order |v|·Δt): derived, d = v½·Δt/2; else
+      // d from the records
       const float fhdt = float(dt * 0.5);
+      float dna[3];
+#pragma unroll
+      for (int d = 0; d < D; ++d) dna[d] = float(xha[d] - xa[d]);
       for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& r) {
         const PartnerH<D>& q = r.h;
         const int32_t ptb = q.pt;
@@ -466,7 +470,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
           const double dh = xha[d] - q.xh[d];
           dx[d] = float(dh);
           dv[d] = vha[d] - q.vh[d];
-          dxn[d] = derive ? dx[d] - dv[d] * fhdt : float(xa[d] - r.xn[d]);
+          dxn[d] = derive ? dx[d] - dv[d] * fhdt : dx[d] - (dna[d] - r.dn[d]);
         }
         // ∇W from pass 1 (positions xⁿ): ∇W = dxn·dwr; terms along ∇W are
         // gathered into one coefficient fg of dxn
