@@ -101,7 +101,9 @@ struct SphBufs {
   int32_t* NC;
   int necap;
   // partner records of the pair kernels, one per sorted particle: PartnerN /
-  // PartnerH(X) below in FP32-split, PartnerN64 / PartnerH64 in FP64
+  // PartnerH(X) below in FP32-split, PartnerN64 / PartnerH64 in FP64.  PRH
+  // holds the half-step records from pass 1 to pass 2, then the XSPH records
+  // (N layout, xⁿ in place of x).
   void* PRN;
   void* PRH;
 };
@@ -180,6 +182,12 @@ struct Records<D, double> {
   using N = PartnerN64<D>;
   using H = PartnerH64<D>;
 };
+// The XSPH pass writes N-layout records into the half-step record buffer.
+static_assert(sizeof(Records<2, float>::H) >= sizeof(Records<2, float>::N) &&
+                  sizeof(Records<3, float>::H) >= sizeof(Records<3, float>::N) &&
+                  sizeof(Records<2, double>::H) >= sizeof(Records<2, double>::N) &&
+                  sizeof(Records<3, double>::H) >= sizeof(Records<3, double>::N),
+              "half-step records hold an N record");
 
 // Device descriptor of one problem: everything a kernel needs, read by each
 // block from the scene it works on.

@@ -348,20 +348,6 @@ __device__ __forceinline__ T ld_rec(const void* base, int64_t b) {
   return r;
 }
 
-// The 16-byte words from `first` on of record b, the earlier words zero (for a
-// reader that needs only the fields at the end of a record).
-template <typename T, int first>
-__device__ __forceinline__ T ld_rec_tail(const void* base, int64_t b) {
-  static_assert(sizeof(T) % 16 == 0 && first * 16 < int(sizeof(T)), "records are whole 16-byte words");
-  const uint4* p = reinterpret_cast<const uint4*>(static_cast<const char*>(base) + b * int64_t(sizeof(T)));
-  uint4 w[sizeof(T) / 16];
-#pragma unroll
-  for (int i = 0; i < int(sizeof(T) / 16); ++i) w[i] = i < first ? make_uint4(0u, 0u, 0u, 0u) : p[i];
-  T r;
-  __builtin_memcpy(&r, w, sizeof(T));
-  return r;
-}
-
 template <typename T>
 __device__ __forceinline__ void st_rec(void* base, int64_t a, const T& r) {
   uint4 w[sizeof(T) / 16];

@@ -436,6 +436,27 @@ __global__ void k_dpc_apply(SS ss) {
   }
 }
 
+// Partner records of the XSPH pass in the Records::N layout: xⁿ (the
+// positions XSPH weighs with), v and ρ after the update and the DPC, ptype.
+// They go over the half-step records, which pass 2 was the last to read.
+template <int D, typename R, typename SS>
+__global__ void k_xsph_rec(SS ss) {
+  GCLSPH_SCENE(ss);
+  if (!F.xsph_on) return;
+  const StepCtl* ctl = B.ctl;
+  const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity;
+  if (a >= B.n) return;
+  typename Records<D, R>::N rec{};
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    rec.x[d] = B.X[p][d][a];
+    rec.v[d] = B.V[q][d][a];
+  }
+  rec.rho = B.RHO[q][a];
+  rec.pt = B.PT[pp][a];
+  st_rec(B.PRH, a, rec);
+}
+
 // xsphcorr! + update_xsphcorr! (src/gpukernels_sph.jl:1757-1838): W at xⁿ.
 template <int D, typename R, typename SS>
 __global__ __launch_bounds__(256) void k_xsph(SS ss) {
@@ -462,36 +483,18 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
     xa[d] = XN[d][a];
     va[d] = V[d][a];
   }
-  // v, ρ and ptype of a partner from its record (updated by the DPC), xⁿ from
-  // the position arrays.  FP32-split: only the record words past the updated
-  // position (which this pass does not use) are loaded.
-  struct Nb {
-    typename Records<D, R>::N r;
-    double xn[3];
-  };
-  using Rn = typename Records<D, R>::N;
-  auto fetch = [&](const int32_t b) {
-    Nb n;
-    if constexpr (std::is_same<R, float>::value) {
-      static_assert(offsetof(Rn, v) >= 16, "v starts in the second record word");
-      n.r = ld_rec_tail<Rn, 1>(B.PRN, b);
-    } else {
-      n.r = ld_rec<Rn>(B.PRN, b);
-    }
-#pragma unroll
-    for (int d = 0; d < D; ++d) n.xn[d] = XN[d][b];
-    return n;
-  };
+  // xⁿ, v, ρ and ptype of a partner from the records of k_xsph_rec
+  using Nb = typename Records<D, R>::N;
+  auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b); };
   double out[3];
   if constexpr (std::is_same<R, double>::value) {
     const double rhoa = RHO[a];
     double sum[3] = {0.0, 0.0, 0.0};
-    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& m) {
-      const auto& n = m.r;
+    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
       if (n.pt <= 1) return;
       double dx[3];
 #pragma unroll
-      for (int d = 0; d < D; ++d) dx[d] = xa[d] - m.xn[d];
+      for (int d = 0; d < D; ++d) dx[d] = xa[d] - n.x[d];
       const double r = sqrt(dotd<D>(dx, dx));
       const double w = W_d(r * C.Hinv, C.wnorm);
       const double xs = C.xsph_c * w / (rhoa + n.rho);
@@ -504,12 +507,11 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
 #pragma clang fp contract(fast)
     const float rhoa = C.f_rho0 + RHO[a];
     Sum<float> sum[3];
-    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& m) {
-      const auto& n = m.r;
+    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
       if (n.pt <= 1) return;
       float dx[3];
 #pragma unroll
-      for (int d = 0; d < D; ++d) dx[d] = float(xa[d] - m.xn[d]);
+      for (int d = 0; d < D; ++d) dx[d] = float(xa[d] - n.x[d]);
       const float r = __builtin_amdgcn_sqrtf(dotf<D>(dx, dx));
       const float w = W_f(r * C.f_Hinv, C.f_wnorm);
       const float xs = C.f_xsph_c * w * __builtin_amdgcn_rcpf(rhoa + (C.f_rho0 + n.rho));
@@ -917,7 +919,10 @@ void launch_dpc(const LaunchSet& L, hipStream_t s) {
 }
 template <int D, typename R>
 void launch_xsph(const LaunchSet& L, hipStream_t s) {
-  with_scenes<R>(L, [&](auto ss) { k_xsph<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss); });
+  with_scenes<R>(L, [&](auto ss) {
+    k_xsph_rec<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss);
+    k_xsph<D, R, decltype(ss)><<<L.pblocks, kBlock, 0, s>>>(ss);
+  });
 }
 template <int D, typename R>
 void launch_cspm(const LaunchSet& L, hipStream_t s) {
