@@ -440,7 +440,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       const float mPa = -C.f_m0 * inv_rhoa;  // -m0/ρ_a of the pressure term
       Pha = B.PH[a];
       // tⁿ separation from the half-step state when x½ = xⁿ + v½·Δt/2, else
-      // from the xⁿ that pass 1 appended to the records
+      // from the displacement x½ - xⁿ that pass 1 appended to the records
       const bool derive = F.synced_prev;
       using Nb = PartnerHX<D>;
       auto fetch = [&](const int32_t b) {
