@@ -115,14 +115,23 @@ struct Rows {
   int32_t b[N], e[N];
 };
 
+// Grid coordinates of cell c (x fastest).  Cell ids are non-negative int32, so
+// 32-bit unsigned division decodes them; in 2D c / gx is already y.
+template <int D>
+__device__ __forceinline__ void cell_xyz(const CellGeom& g, int32_t c, int64_t& ix, int64_t& iy, int64_t& iz) {
+  const uint32_t ugx = uint32_t(g.grid[0]), ugy = uint32_t(g.grid[1]), uc = uint32_t(c);
+  const uint32_t rest = uc / ugx;
+  ix = int64_t(uc - rest * ugx);
+  iy = D == 3 ? int64_t(rest % ugy) : int64_t(rest);
+  iz = D == 3 ? int64_t(rest / ugy) : 0;
+}
+
 template <int D>
 __device__ __forceinline__ Rows<D> stencil_rows(const CellGeom& g, const int32_t* __restrict__ start, int32_t c) {
   Rows<D> r;
   const int64_t gx = g.grid[0], gy = g.grid[1];
-  const int64_t ix = c % gx;
-  const int64_t rest = c / gx;
-  const int64_t iy = rest % gy;
-  const int64_t iz = rest / gy;
+  int64_t ix, iy, iz;
+  cell_xyz<D>(g, c, ix, iy, iz);
   const int64_t xl = ix > 0 ? ix - 1 : 0, xh = ix + 1 < gx ? ix + 1 : gx - 1;
 #pragma unroll
   for (int k = 0; k < Rows<D>::N; ++k) {

@@ -314,8 +314,8 @@ template <int D>
 __device__ __forceinline__ void row_range(const CellGeom& g, const int32_t* __restrict__ start, int32_t c, int k,
                                           int32_t& b, int32_t& e) {
   const int64_t gx = g.grid[0], gy = g.grid[1];
-  const int64_t ix = c % gx, rest = c / gx;
-  const int64_t iy = rest % gy, iz = rest / gy;
+  int64_t ix, iy, iz;
+  cell_xyz<D>(g, c, ix, iy, iz);
   const int64_t xl = ix > 0 ? ix - 1 : 0, xh = ix + 1 < gx ? ix + 1 : gx - 1;
   const int64_t y = iy + (k % 3) - 1;
   const int64_t z = D == 3 ? iz + (k / 3) - 1 : 0;
@@ -362,20 +362,20 @@ __device__ __forceinline__ typename Records<D, R>::N* rec_n(void* base) {
   return static_cast<typename Records<D, R>::N*>(base);
 }
 
-// The union of the stencil rows of one block in one plane (2D: the only plane;
-// 3D: the plane dz ∈ {-1, 0, 1}).  The block's consecutive particles a0..a1
-// occupy one contiguous range of cells [c0, c1], so for each row offset k the
-// cells [c0 + o - 1, c1 + o + 1], o = (k-1)·gx + dz·gx·gy, hold every candidate
-// of that row of every particle of the block: three contiguous ranges of
-// sorted particles starting at ub[k], laid out one after the other from LDS
-// slot uo[k].
+// The union of the stencil rows of `span` consecutive particles from a0 (a
+// block or a wave) in one plane (2D: the only plane; 3D: the plane dz ∈ {-1,
+// 0, 1}).  The particles a0..a1 occupy one contiguous range of cells [c0, c1],
+// so for each row offset k the cells [c0 + o - 1, c1 + o + 1], o = (k-1)·gx +
+// dz·gx·gy, hold every candidate of that row of every one of them: three
+// contiguous ranges of sorted particles starting at ub[k], laid out one after
+// the other from LDS slot uo[k].
 struct RowUnion {
   int32_t ub[3], uo[4];
 };
 
 __device__ __forceinline__ void row_union(RowUnion& u, const int32_t* start, const int32_t* skey, const CellGeom& G,
-                                          int64_t n, int64_t a0, int dz = 0) {
-  const int64_t a1 = (a0 + kBlock < n ? a0 + kBlock : n) - 1;
+                                          int64_t n, int64_t a0, int dz = 0, int span = kBlock) {
+  const int64_t a1 = (a0 + span < n ? a0 + span : n) - 1;
   const int64_t c0 = skey[a0], c1 = skey[a1];
   const int64_t gx = G.grid[0], nc = G.ncells();
   const int64_t zo = int64_t(dz) * gx * G.grid[1];

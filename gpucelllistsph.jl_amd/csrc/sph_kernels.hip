@@ -93,10 +93,10 @@ __device__ __forceinline__ uint32_t window_bits_lds3(const double* lx, const dou
   return window_mask(acc, len, a, lo);
 }
 
-// Candidates staged in LDS per block: 2D 2036 positions, 3D 1280 per plane of
-// three stencil rows; with the padding and the row union each fits in 32 KiB,
-// which keeps five blocks per CU.
-constexpr int kNmaskLds = 2036;
+// Candidates staged in LDS: 2D 400 positions per wave (the union of a wave's
+// three stencil rows holds ~300 on a uniform lattice), 26 KiB per block, six
+// blocks per CU; 3D 1280 per block and plane of three stencil rows, 32 KiB.
+constexpr int kNmaskWave = 400;
 constexpr int kNmaskPlane = 1280;
 
 // Neighbour-list membership of the last rebuild, stored per particle as
@@ -106,13 +106,14 @@ constexpr int kNmaskPlane = 1280;
 // entry count, or -1 when the particle needs more than necap entries (its
 // partners are then found by scanning the stencil).
 //
-// 2D: the block stages the positions of its row union (row_union) in LDS once
-// (coalesced) and every lane tests its candidates there, instead of each lane
-// fetching each of its ~100 candidates through the vector L1.  3D: the same
-// plane by plane (three stencil rows at a time, ~1000 candidates per particle
-// in all).  A block whose union exceeds the LDS budget reads the candidates
-// from global memory (per plane in 3D).  The tests are the same expression in
-// every path, so the bits are the same.
+// 2D: each wave stages the positions of its own row union (row_union over its
+// 64 particles) in its LDS slice (coalesced) and every lane tests its
+// candidates there, instead of each lane fetching each of its ~100 candidates
+// through the vector L1; waves do not wait for each other.  3D: the block
+// stages plane by plane (three stencil rows at a time, ~1000 candidates per
+// particle in all).  A wave (2D) or block (3D) whose union exceeds the LDS
+// budget reads the candidates from global memory.  The tests are the same
+// expression in every path, so the bits are the same.
 template <int D, typename R, typename SS>
 __global__ __launch_bounds__(256) void k_nmask(SS ss) {
   GCLSPH_SCENE(ss);
@@ -127,27 +128,35 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
   const int cap = B.necap;
   uint64_t* out = B.NE + a;
   if constexpr (D == 2) {
-    __shared__ double2 lds[kNmaskLds + 4];  // +4: window reads run up to 3 past a row's end
-    __shared__ RowUnion u;
-    if (threadIdx.x == 0) row_union(u, B.start, B.skey, G, B.n, a0);
-    __syncthreads();
-    const int32_t* ub = u.ub;
-    const int32_t* uo = u.uo;
-    const int32_t total = uo[3];
-    if (total <= kNmaskLds) {  // uniform over the block
-      for (int32_t i = threadIdx.x; i < total; i += kBlock) {
-        const int k = (i >= uo[1]) + (i >= uo[2]);
-        const int32_t b = ub[k] + (i - uo[k]);
-        lds[i] = make_double2(XM[0][b], XM[1][b]);
+    __shared__ double2 lds[kBlock / 64][kNmaskWave + 4];  // +4: window reads run up to 3 past a row's end
+    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+    const int lane = int(threadIdx.x & 63);
+    const int64_t aw = a0 + 64 * w;
+    if (aw >= B.n) return;  // whole wave; no block barrier follows
+    RowUnion u;
+    row_union(u, B.start, B.skey, G, B.n, aw, 0, 64);
+    const int32_t total = u.uo[3];
+    if (total <= kNmaskWave) {  // uniform over the wave
+      double2* wl = lds[w];
+      // slot i holds candidate i + d_k of row k (selects, not an indexed
+      // array: the union lives in registers)
+      const int32_t d0 = u.ub[0] - u.uo[0], d1 = u.ub[1] - u.uo[1], d2 = u.ub[2] - u.uo[2];
+      for (int32_t i = lane; i < total; i += 64) {
+        const int32_t b = i + (i >= u.uo[2] ? d2 : (i >= u.uo[1] ? d1 : d0));
+        wl[i] = make_double2(XM[0][b], XM[1][b]);
       }
-      __syncthreads();
+      // the wave's LDS operations complete in issue order, so its lanes read
+      // what the others wrote above; the fence keeps the compiler from moving
+      // the reads ahead of the writes
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
       if (!active) return;
       const double xa0 = XM[0][a], xa1 = XM[1][a];
       const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
       int cnt = 0;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double2* lrow = lds + (uo[k] - ub[k]);  // candidate b at lrow[b]
+        const double2* lrow = wl + (u.uo[k] - u.ub[k]);  // candidate b at lrow[b]
         for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
           const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
           const uint32_t mk = window_bits_lds(lrow + w0, xa0, xa1, a, w0, hi, dist2);
