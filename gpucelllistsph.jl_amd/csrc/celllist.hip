@@ -71,8 +71,16 @@ __global__ void k_rank(const int32_t* __restrict__ key, const int32_t* __restric
   int32_t i = tmp[s];
   int32_t c = key[i];
   int32_t b = start[c], e = start[c + 1];
+  // four loads in flight per trip (indices clamped into the cell, which holds
+  // at least this item; the clamped ones are not counted)
   int32_t r = 0;
-  for (int32_t k = b; k < e; ++k) r += tmp[k] < i;
+  for (int32_t k = b; k < e; k += 4) {
+    int32_t v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = tmp[k + u < e ? k + u : e - 1];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r += (k + u < e) & (v[u] < i);
+  }
   perm[b + r] = i;
   skey[b + r] = c;
 }

@@ -892,8 +892,16 @@ __global__ void kb_rank(SS ss) {
   const int32_t i = sd.tmp[a];
   const int32_t c = sd.key[i];
   const int32_t b = sd.start[c], e = sd.start[c + 1];
+  // four loads in flight per trip (indices clamped into the cell, which holds
+  // at least this particle; the clamped ones are not counted)
   int32_t r = 0;
-  for (int32_t k = b; k < e; ++k) r += sd.tmp[k] < i;
+  for (int32_t k = b; k < e; k += 4) {
+    int32_t v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = sd.tmp[k + u < e ? k + u : e - 1];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r += (k + u < e) & (v[u] < i);
+  }
   sd.perm[b + r] = i;
   sd.skey[b + r] = c;
 }
