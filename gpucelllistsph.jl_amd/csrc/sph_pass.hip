@@ -227,12 +227,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 #pragma unroll
       for (int d = 0; d < D; ++d) ax[d] += fg * dx[d];
       if ((pta > 0) != (ptb > 0) && r < C.f_fb_r0[0]) {
-        // wall force in FP64: (r0/r)^12 overflows FP32 for near-coincident pairs
-        double q1 = C.fb_r0[0] / (double(r) + C.fb_eta2[0]);
-        double q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
-        double Fc = C.fb_d[0] * (q4 * q8 - q4) / (double(r2) + C.fb_eta2[0]);
+        // wall force: FP32 while q = r0/(r + η²) < 32 (q^12 < 1.2e18, far inside
+        // the FP32 range), FP64 for the (rare) closer pairs
+        const float q1 = C.f_fb_r0[0] * __builtin_amdgcn_rcpf(r + C.f_fb_eta2[0]);
+        if (q1 < 32.0f) {
+          const float q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
+          const float Fc = C.f_fb_d[0] * (q4 * q8 - q4) * __builtin_amdgcn_rcpf(r2 + C.f_fb_eta2[0]);
 #pragma unroll
-        for (int d = 0; d < D; ++d) ax[d] += float(double(dx[d]) * Fc);
+          for (int d = 0; d < D; ++d) ax[d] += dx[d] * Fc;
+        } else {
+          double p1 = C.fb_r0[0] / (double(r) + C.fb_eta2[0]);
+          double p2 = p1 * p1, p4 = p2 * p2, p8 = p4 * p4;
+          double Fc = C.fb_d[0] * (p4 * p8 - p4) / (double(r2) + C.fb_eta2[0]);
+#pragma unroll
+          for (int d = 0; d < D; ++d) ax[d] += float(double(dx[d]) * Fc);
+        }
       }
 #pragma unroll
       for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
@@ -521,12 +530,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 #pragma unroll
         for (int d = 0; d < D; ++d) ax[d] += fg * dxn[d];
         if ((pta > 0) != (ptb > 0) && rn < C.f_fb_r0[1]) {
-          // wall force in FP64: (r0/r)^12 overflows FP32 for near-coincident pairs
-          double q1 = C.fb_r0[1] / (double(rn) + C.fb_eta2[1]);
-          double q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
-          double Fc = C.fb_d[1] * (q4 * q8 - q4) / (double(r2n) + C.fb_eta2[1]);
+          // wall force: FP32 while q < 32, FP64 for closer pairs (as in pass 1)
+          const float q1 = C.f_fb_r0[1] * __builtin_amdgcn_rcpf(rn + C.f_fb_eta2[1]);
+          if (q1 < 32.0f) {
+            const float q2 = q1 * q1, q4 = q2 * q2, q8 = q4 * q4;
+            const float Fc = C.f_fb_d[1] * (q4 * q8 - q4) * __builtin_amdgcn_rcpf(r2n + C.f_fb_eta2[1]);
 #pragma unroll
-          for (int d = 0; d < D; ++d) ax[d] += float(double(dxn[d]) * Fc);
+            for (int d = 0; d < D; ++d) ax[d] += dxn[d] * Fc;
+          } else {
+            double p1 = C.fb_r0[1] / (double(rn) + C.fb_eta2[1]);
+            double p2 = p1 * p1, p4 = p2 * p2, p8 = p4 * p4;
+            double Fc = C.fb_d[1] * (p4 * p8 - p4) / (double(r2n) + C.fb_eta2[1]);
+#pragma unroll
+            for (int d = 0; d < D; ++d) ax[d] += float(double(dxn[d]) * Fc);
+          }
         }
 #pragma unroll
         for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
