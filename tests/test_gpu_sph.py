@@ -353,3 +353,33 @@ def test_pass_variants_bit_identical(G, monkeypatch, prec):
         got = run(v)
         for a, b in zip(got, ref):
             assert np.array_equal(np.asarray(a), np.asarray(b)), v
+
+
+def test_wall_force_fp32_and_fp64_branches(G, O, S):
+    """FP32-split evaluates the wall force in FP32 while q = r0/(r + η²) < 32 and
+    in FP64 for closer pairs.  One fluid particle 0.02·dx above a wall particle
+    (q ≈ 96, FP64 branch) and one 0.9·dx above another (q ≈ 2.2, FP32 branch):
+    FP32-split accelerations within 1e-5 of FP64 per particle, FP64 against the
+    oracle.  Δt = 1e-20 keeps the half step from moving the particles."""
+    dx = 0.02
+    p, dist = S.standard_params(dx, 2)
+    p.dt = 1e-20
+    wx = np.arange(10) * dx
+    x = np.concatenate([wx, [2 * dx, 6 * dx]])
+    y = np.concatenate([np.zeros(10), [0.02 * dx, 0.9 * dx]])
+    ptype = np.array([0] * 10 + [2, 2], dtype=np.int32)
+    rho = np.full(12, p.rho0)
+    sc = S.Scene(2, (x, y), rho, ptype, p, dist, (dist, dist))
+    r0, eta2 = 2 * dx, 0.01 * (2 * dx) ** 2
+    assert r0 / (0.02 * dx + eta2) > 32 > r0 / (0.9 * dx + eta2)
+    cl, oc, p64, ref = _pair(G, O, sc)
+    cl32 = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+    p32 = G.SPHProblem.from_params(cl32, sc.params, sc.rho, sc.ptype, precision="f32split")
+    G.stepsolve_(p64, 1)
+    G.stepsolve_(p32, 1)
+    ref.stepsolve(1)
+    a64, a32 = np.array(G.get_acceleration(p64)), np.array(G.get_acceleration(p32))
+    assert rel_err(a64, ref.get("acc")) < 1e-9
+    for i in (10, 11):
+        assert np.linalg.norm(a32[:, i] - a64[:, i]) < 1e-5 * np.linalg.norm(a64[:, i]), i
+    assert np.linalg.norm(a64[:, 10]) > 1e6 * np.linalg.norm(a64[:, 11])  # the close pair dominates
