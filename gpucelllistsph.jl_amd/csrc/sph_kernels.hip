@@ -858,21 +858,33 @@ __global__ void kb_tile_scan(SS ss) {
   const int64_t n = sd.ncells, nt = (n + kTile - 1) / kTile;
   if (m.y >= nt) return;
   const int64_t base = int64_t(m.y) * kTile + int64_t(threadIdx.x) * kScanItems;
+  static_assert(kScanItems == 4, "one 16-byte word of counts per thread");
+  const bool whole = base + kScanItems <= n;  // the last tile's tail goes element by element
   int32_t v[kScanItems];
+  if (whole) {
+    const int4 w = *reinterpret_cast<const int4*>(sd.cnt + base);
+    v[0] = w.x, v[1] = w.y, v[2] = w.z, v[3] = w.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) v[k] = base + k < n ? sd.cnt[base + k] : 0;
+  }
   int64_t s = 0;
 #pragma unroll
-  for (int k = 0; k < kScanItems; ++k) {
-    const int64_t i = base + k;
-    v[k] = i < n ? sd.cnt[i] : 0;
-    s += v[k];
-  }
+  for (int k = 0; k < kScanItems; ++k) s += v[k];
   int64_t tot;
   int64_t pre = block_exclusive_scan<int64_t>(s, &tot) + sd.scan_tmp[m.y];
+  int32_t o[kScanItems];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    const int64_t i = base + k;
-    if (i < n) sd.start[i] = int32_t(pre);
+    o[k] = int32_t(pre);
     pre += v[k];
+  }
+  if (whole) {
+    *reinterpret_cast<int4*>(sd.start + base) = make_int4(o[0], o[1], o[2], o[3]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+      if (base + k < n) sd.start[base + k] = o[k];
   }
   if (m.y == 0 && threadIdx.x == 0) sd.start[n] = int32_t(sd.scan_tmp[nt]);
 }
