@@ -137,6 +137,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
+    ap.add_argument("--scene", choices=("2d", "3d", "ensemble"), default="2d",
+                    help="2d: the headline line (default); 3d: the 883,376-particle 3D dam break with every "
+                         "physics term, FP32-split and FP64; ensemble: only the batched-ensemble leg")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pass2_f32split_traffic.txt"))
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="copy a buffer of this many GiB three times and exit (HBM counter calibration)")
@@ -157,6 +160,21 @@ def main():
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    if args.scene == "ensemble":
+        print(json.dumps({"metric": "particle-steps/s (batched ensemble)", "ensemble": run_ensemble(args.steps,
+                                                                                                     args.warmup)}))
+        return
+    if args.scene == "3d":
+        sc3 = S.config2(dx=0.01)
+        r = {prec: run_gpu(sc3, prec, args.steps, args.warmup) for prec in ("f32split", "f64")}
+        print(json.dumps({
+            "metric": "particle-steps/s (full SPH step, 3D dam break)",
+            "config": {"workload": "3D dam break dx=0.01: 1.0x1.0x0.5 m block in a 3.2x1.5x0.5 m tank, nu, "
+                                   "surface tension, DPC, XSPH, CSPM, adaptive CFL", "particles": sc3.n},
+            **{prec: {"value": sc3.n * args.steps / v["seconds"], "ms_per_step": v["seconds"] / args.steps * 1e3,
+                      "stage_ms_per_step": {k: t[0] / args.steps for k, t in v["timing"].items() if t[1] > 0}}
+               for prec, v in r.items()}}))
+        return
     scene = S.config3(dx=args.dx)
     n = scene.n
     res32 = run_gpu(scene, "f32split", args.steps, args.warmup)
