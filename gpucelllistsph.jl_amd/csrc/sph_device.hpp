@@ -216,10 +216,21 @@ __device__ __forceinline__ const SceneDev<R>& particle_scene(const OneScene<R>& 
   m = make_int2(0, int(xcd_block()));
   return s.d;
 }
+// Descriptor i of a batch, read through the constant address space: the
+// descriptors are not written while a kernel runs, so their loads are
+// invariant (scalar loads the compiler may repeat at will, as for a kernel
+// argument), and the buffer pointers read from them are used as global-memory
+// pointers instead of falling back to flat addressing.
+template <typename R>
+__device__ __forceinline__ const SceneDev<R>& batch_scene(const ManyScenes<R>& s, int i) {
+  using CS = const __attribute__((address_space(4))) SceneDev<R>;
+  CS* c = (CS*)(s.S);
+  return *(const SceneDev<R>*)(c + i);
+}
 template <typename R>
 __device__ __forceinline__ const SceneDev<R>& particle_scene(const ManyScenes<R>& s, int2& m) {
   m = s.pmap[xcd_block()];
-  return s.S[m.x];
+  return batch_scene(s, m.x);
 }
 template <typename R>
 __device__ __forceinline__ const SceneDev<R>& tile_scene(const OneScene<R>& s, int2& m) {
@@ -229,7 +240,7 @@ __device__ __forceinline__ const SceneDev<R>& tile_scene(const OneScene<R>& s, i
 template <typename R>
 __device__ __forceinline__ const SceneDev<R>& tile_scene(const ManyScenes<R>& s, int2& m) {
   m = s.cmap[blockIdx.x];
-  return s.S[m.x];
+  return batch_scene(s, m.x);
 }
 template <typename R>
 __device__ __forceinline__ const SceneDev<R>& scene_at(const OneScene<R>& s, unsigned) {
@@ -237,7 +248,7 @@ __device__ __forceinline__ const SceneDev<R>& scene_at(const OneScene<R>& s, uns
 }
 template <typename R>
 __device__ __forceinline__ const SceneDev<R>& scene_at(const ManyScenes<R>& s, unsigned i) {
-  return s.S[i];
+  return batch_scene(s, int(i));
 }
 
 // Per-step switches of a problem, from its per-call switches and control block.
