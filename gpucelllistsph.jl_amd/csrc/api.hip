@@ -1045,7 +1045,7 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
     GCL_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
     for (int k = 0; k < dim; ++k) {
       GCL_HIP(hipMalloc(&g->xo[k], g->tsize() * size_t(n)));
-      GCL_HIP(hipMalloc(&g->xs[k], g->tsize() * size_t(n)));
+      GCL_HIP(hipMalloc(&g->xs[k], g->tsize() * size_t(n + kPosPad)));  // pair windows read past the end
     }
     g->upload_points(pts);
     g->sb.alloc(n, g->G.ncells());

@@ -95,6 +95,43 @@ __global__ void k_gather_pos(XPtrs xp, XPtrs xsp, const int32_t* __restrict__ pe
     const_cast<T*>(static_cast<const T*>(xsp.p[k]))[s] = static_cast<const T*>(xp.p[k])[i];
 }
 
+// Partner bits of the window [lo, hi) (hi - lo <= 32) of item s: bit j is set
+// when r^2 < dist2max for candidate lo + j, with the expression of dist2_exact
+// (no FMA).  Candidates are read four at a time so their loads are in flight
+// together; reads run up to 3 past hi (the position arrays carry a padding of
+// kPosPad elements), and those bits are dropped.
+template <typename T, int D>
+__device__ __forceinline__ uint32_t pair_window(const T* const* x, T xa0, T xa1, T xa2, int32_t lo, int32_t hi,
+                                                T dist2max) {
+#pragma clang fp contract(off)
+  const int32_t len = hi - lo;
+  uint32_t mk = 0;
+  for (int32_t j = 0; j < len; j += 4) {
+    T c0[4], c1[4], c2[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      c0[u] = x[0][lo + j + u];
+      c1[u] = x[1][lo + j + u];
+      if (D == 3) c2[u] = x[2][lo + j + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const T d0 = xa0 - c0[u];
+      const T d1 = xa1 - c1[u];
+      T r2 = d0 * d0 + d1 * d1;
+      if (D == 3) {
+        const T d2 = xa2 - c2[u];
+        r2 = r2 + d2 * d2;
+      }
+      mk |= uint32_t(r2 < dist2max) << (j + u);
+    }
+  }
+  return len < 32 ? mk & ((1u << len) - 1u) : mk;
+}
+
+// Neighbours t > s of sorted item s, stencil row by row in 32-candidate
+// windows (ascending t inside a row): count pass, then the emit pass writes
+// them in the same order.
 template <typename T, int D>
 __global__ void k_pair_count(XPtrs xsp, const int32_t* __restrict__ start, const int32_t* __restrict__ skey,
                              int64_t n, CellGeom g, T dist2max, int32_t* __restrict__ cnt) {
@@ -102,42 +139,69 @@ __global__ void k_pair_count(XPtrs xsp, const int32_t* __restrict__ start, const
   if (s >= n) return;
   const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
                    static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
+  const T xa0 = x[0][s], xa1 = x[1][s], xa2 = D == 3 ? x[2][s] : T(0);
   const Rows<D> r = stencil_rows<D>(g, start, skey[s]);
   int32_t c = 0;
 #pragma unroll
   for (int k = 0; k < Rows<D>::N; ++k) {
-    int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
-    for (int32_t t = b; t < r.e[k]; ++t) c += dist2_exact<T, D>(x, s, t) < dist2max;
+    const int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
+    for (int32_t w0 = b; w0 < r.e[k]; w0 += 32) {
+      const int32_t hi = r.e[k] < w0 + 32 ? r.e[k] : w0 + 32;
+      c += __builtin_popcount(pair_window<T, D>(x, xa0, xa1, xa2, w0, hi, dist2max));
+    }
   }
   cnt[s] = c;
 }
 
+// Pairs staged in LDS per wave before the wave writes them out: a wave's
+// items are consecutive, so its pairs fill one contiguous range of the output
+// [off[s0], off[s0 + 64]); staged, that range goes out in whole-line stores
+// instead of one scattered 8-byte store per pair.  A wave with more pairs
+// writes the rest directly.  40 KiB per block.
+constexpr int kEmitStage = 1280;
+
 template <typename T, int D>
-__global__ void k_pair_emit(XPtrs xsp, const int32_t* __restrict__ start, const int32_t* __restrict__ skey,
-                            const int32_t* __restrict__ id, int64_t n, CellGeom g, T dist2max,
-                            const int64_t* __restrict__ off, int32_t* __restrict__ pairs, int64_t capacity) {
-  int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  int64_t o = off[s];
-  if (o >= capacity) return;
-  const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
-                   static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
-  const Rows<D> r = stencil_rows<D>(g, start, skey[s]);
-  int32_t is = id[s] + 1;
+__global__ __launch_bounds__(256) void k_pair_emit(XPtrs xsp, const int32_t* __restrict__ start,
+                                                   const int32_t* __restrict__ skey, const int32_t* __restrict__ id,
+                                                   int64_t n, CellGeom g, T dist2max, const int64_t* __restrict__ off,
+                                                   int32_t* __restrict__ pairs, int64_t capacity) {
+  __shared__ int2 stage[kBlock / 64][kEmitStage];
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int64_t s0 = int64_t(blockIdx.x) * kBlock + 64 * w;
+  if (s0 >= n) return;  // whole wave; only wave-level ordering follows
+  const int64_t s = s0 + (threadIdx.x & 63);
+  const int64_t o0 = off[s0];
+  const int64_t oend = off[s0 + 64 < n ? s0 + 64 : n];
+  int2* st = stage[w];
+  int2* out = reinterpret_cast<int2*>(pairs);
+  if (s < n) {
+    int64_t o = off[s];
+    const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
+                     static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
+    const T xa0 = x[0][s], xa1 = x[1][s], xa2 = D == 3 ? x[2][s] : T(0);
+    const Rows<D> r = stencil_rows<D>(g, start, skey[s]);
+    const int32_t is = id[s] + 1;
 #pragma unroll
-  for (int k = 0; k < Rows<D>::N; ++k) {
-    int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
-    for (int32_t t = b; t < r.e[k]; ++t) {
-      if (dist2_exact<T, D>(x, s, t) < dist2max) {
-        if (o < capacity) {
-          int32_t it = id[t] + 1;
-          int2 pr = is < it ? make_int2(is, it) : make_int2(it, is);
-          reinterpret_cast<int2*>(pairs)[o] = pr;
+    for (int k = 0; k < Rows<D>::N; ++k) {
+      const int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
+      for (int32_t w0 = b; w0 < r.e[k] && o < capacity; w0 += 32) {
+        const int32_t hi = r.e[k] < w0 + 32 ? r.e[k] : w0 + 32;
+        uint32_t mk = pair_window<T, D>(x, xa0, xa1, xa2, w0, hi, dist2max);
+        for (; mk != 0 && o < capacity; mk &= mk - 1, ++o) {
+          const int32_t it = id[w0 + __builtin_ctz(mk)] + 1;
+          const int2 pr = is < it ? make_int2(is, it) : make_int2(it, is);
+          if (o - o0 < kEmitStage) st[o - o0] = pr;
+          else out[o] = pr;
         }
-        ++o;
       }
     }
   }
+  // the wave's LDS operations complete in issue order, so its lanes read what
+  // the others staged; the fence keeps the compiler from hoisting the reads
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  const int64_t ns = oend - o0 < kEmitStage ? oend - o0 : kEmitStage;
+  for (int64_t i = threadIdx.x & 63; i < ns && o0 + i < capacity; i += 64) out[o0 + i] = st[i];
 }
 
 template <typename T>
@@ -149,9 +213,9 @@ void bin_sort_t(const CellGeom& g, const void* const* x, int64_t n, bool use_off
   int64_t nc = g.ncells();
   fill_i32(sb.cnt, nc + 1, 0, flag, s);
   if (g.dim == 2)
-    k_cellmap<T, 2><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, sel != nullptr);
+    k_cellmap<T, 2><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, 1);
   else
-    k_cellmap<T, 3><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, sel != nullptr);
+    k_cellmap<T, 3><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, 1);
   GCL_HIP(hipGetLastError());
   fill_i32(sb.maxcell, 1, 0, flag, s);
   exclusive_scan_i32(sb.cnt, sb.start, nc, sb.scan_tmp, flag, s, sb.maxcell);
