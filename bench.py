@@ -137,9 +137,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-f64", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
-    ap.add_argument("--scene", choices=("2d", "3d", "ensemble"), default="2d",
+    ap.add_argument("--scene", choices=("2d", "3d", "ensemble", "celllist"), default="2d",
                     help="2d: the headline line (default); 3d: the 883,376-particle 3D dam break with every "
-                         "physics term, FP32-split and FP64; ensemble: only the batched-ensemble leg")
+                         "physics term, FP32-split and FP64; ensemble: only the batched-ensemble leg; celllist: "
+                         "update! (binning + pair list) of a cell list over the 4 M-particle scene")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pass2_f32split_traffic.txt"))
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="copy a buffer of this many GiB three times and exit (HBM counter calibration)")
@@ -160,6 +161,20 @@ def main():
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    if args.scene == "celllist":
+        sc = S.config3(dx=args.dx)
+        cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        for _ in range(args.warmup):
+            G.update_(cl)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            G.update_(cl)
+        dt = (time.perf_counter() - t0) / args.steps
+        print(json.dumps({"metric": "cell-list update! (binning + pair list) per second", "value": 1.0 / dt,
+                          "unit": "updates/s", "ms_per_update": dt * 1e3, "steps": args.steps,
+                          "config": {"workload": f"GPUCellList over the 2D dam break dx={args.dx}, FP64 points, "
+                                                 "cell size = dist = H", "particles": sc.n, "pairs": cl.pairsn}}))
+        return
     if args.scene == "ensemble":
         print(json.dumps({"metric": "particle-steps/s (batched ensemble)", "ensemble": run_ensemble(args.steps,
                                                                                                      args.warmup)}))
