@@ -35,10 +35,13 @@ struct XPtrs {
 template <typename T, int D>
 __global__ void k_cellmap(XPtrs xp, XPtrs xalt, const int* sel, int64_t n, CellGeom g, int use_off64,
                           int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ cnt,
-                          const int* flag, int aggregate) {
+                          const int* flag, int aggregate, const int32_t* __restrict__ order) {
   if (flag && *flag == 0) return;
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  // items in the previous binning's order when there is one: neighbouring
+  // threads then mostly share a cell, which the wave-aggregated slots need
+  const int64_t i = order ? int64_t(order[t]) : t;
   const XPtrs& src = (sel && *sel) ? xalt : xp;
   int32_t c[3] = {1, 1, 1};
 #pragma unroll
@@ -213,9 +216,11 @@ void bin_sort_t(const CellGeom& g, const void* const* x, int64_t n, bool use_off
   int64_t nc = g.ncells();
   fill_i32(sb.cnt, nc + 1, 0, flag, s);
   if (g.dim == 2)
-    k_cellmap<T, 2><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, 1);
+    k_cellmap<T, 2><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, 1,
+                                                   sb.sorted ? sb.perm : nullptr);
   else
-    k_cellmap<T, 3><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, 1);
+    k_cellmap<T, 3><<<grid_for(n), kBlock, 0, s>>>(xp, xa, sel, n, g, use_off64 ? 1 : 0, sb.key, sb.slot, sb.cnt, flag, 1,
+                                                   sb.sorted ? sb.perm : nullptr);
   GCL_HIP(hipGetLastError());
   fill_i32(sb.maxcell, 1, 0, flag, s);
   exclusive_scan_i32(sb.cnt, sb.start, nc, sb.scan_tmp, flag, s, sb.maxcell);
@@ -223,12 +228,14 @@ void bin_sort_t(const CellGeom& g, const void* const* x, int64_t n, bool use_off
   GCL_HIP(hipGetLastError());
   k_rank<<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.start, sb.tmp, sb.perm, sb.skey, n, flag);
   GCL_HIP(hipGetLastError());
+  if (!flag) sb.sorted = true;  // perm now holds a permutation of [0, n) (a flagged sort may not have run)
 }
 
 }  // namespace
 
 void SortBuffers::alloc(int64_t n_, int64_t ncells_) {
   release();
+  sorted = false;
   n = n_;
   ncells = ncells_;
   GCL_HIP(hipMalloc(&key, sizeof(int32_t) * size_t(n)));

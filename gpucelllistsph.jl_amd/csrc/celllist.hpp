@@ -88,6 +88,7 @@ struct SortBuffers {
   int32_t* skey = nullptr;   // n: cell of each sorted position
   int64_t* scan_tmp = nullptr;
   int32_t* maxcell = nullptr;  // 1: max particles per cell of the last binning
+  bool sorted = false;         // perm holds a previous binning's order
   void alloc(int64_t n_, int64_t ncells_);
   void release();
 };
