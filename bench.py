@@ -239,7 +239,8 @@ def main():
             "traffic": traffic,
             "bytes_per_particle": bytes_pp[dom],
             "valu_busy": prof.get("valu_busy_per_dispatch"),
-            "note": "pair passes are VALU-bound: achieved = algorithmic bytes / measured kernel time; "
+            "note": "pair passes are co-limited by vector-L1 access rate and VALU issue, not HBM: "
+                    "achieved = algorithmic bytes / measured kernel time; "
                     "valu_busy (share of SIMD cycles issuing VALU) and traffic come from the committed PMC profile",
         },
     }
