@@ -153,20 +153,23 @@ struct gcl_s {
     return total;
   }
 
-  void zero_pairs() {
-    if (pb.capacity > 0) GCL_HIP(hipMemsetAsync(pb.pairs, 0, 8 * size_t(pb.capacity), stream));
-  }
-
   // update! (src/celllist.jl:124-163) for a standalone cell list.
   void update(bool fillzero) {
     bin_sort(G, prec, xo, n, false, sb, nullptr, stream);
     int32_t maxpoint = read_maxcell();
     if (maxpoint > mppcell) mppcell = int64_t(std::ceil(double(maxpoint) * 1.05 + 1));
     gather_positions(dim, prec, xo, xs, sb.perm, n, stream);
-    if (pairsn > pb.capacity || double(pairsn) < double(pb.capacity) * 0.6) pb.set_capacity(ceil_div08(pairsn), stream);
-    else if (fillzero) zero_pairs();
+    bool fresh = false;  // a reallocated buffer is zero-filled whole
+    if (pairsn > pb.capacity || double(pairsn) < double(pb.capacity) * 0.6) {
+      pb.set_capacity(ceil_div08(pairsn), stream);
+      fresh = true;
+    }
     int64_t total = count_pairs(xs);
-    if (total > pb.capacity) pb.set_capacity(ceil_div08(total), stream);
+    if (total > pb.capacity) {
+      pb.set_capacity(ceil_div08(total), stream);
+      fresh = true;
+    }
+    if (fillzero && !fresh) pb.zero_from(total, stream);
     pair_emit(G, prec, xs, sb.start, sb.skey, sb.perm, n, dist, pb, stream);
     pairsn = total;
     GCL_HIP(hipStreamSynchronize(stream));
@@ -174,9 +177,9 @@ struct gcl_s {
 
   // partialupdate! (src/celllist.jl:170-174): current points, cells of the last update!.
   void partialupdate(bool fillzero) {
-    if (fillzero) zero_pairs();
     gather_positions(dim, prec, xo, xs, sb.perm, n, stream);
     int64_t total = count_pairs(xs);
+    if (fillzero) pb.zero_from(total, stream);
     pair_emit(G, prec, xs, sb.start, sb.skey, sb.perm, n, dist, pb, stream);
     pairsn = total;
     GCL_HIP(hipStreamSynchronize(stream));
@@ -963,7 +966,7 @@ void ensure_pairs(gcl_s* g) {
   if (g->pairsn > cap || double(g->pairsn) < double(cap) * 0.6) cap = ceil_div08(g->pairsn);
   if (total > cap) cap = ceil_div08(total);
   if (cap != g->pb.capacity) g->pb.set_capacity(cap, P->stream);
-  else if (cap > 0) GCL_HIP(hipMemsetAsync(g->pb.pairs, 0, 8 * size_t(cap), P->stream));
+  else g->pb.zero_from(total, P->stream);
   pair_emit(P->G, 64, xm, P->sb.start, P->sb.skey, P->idcur(), g->n, g->dist, g->pb, P->stream);
   GCL_HIP(hipStreamSynchronize(P->stream));
   g->pairsn = total;
@@ -1107,12 +1110,12 @@ int gcl_partialupdate(gcl_t h, int fillzero) {
     if (h->attached) {
       sph_s* P = h->attached;
       P->read_ctl();
-      if (fillzero && h->pb.capacity > 0)
-        GCL_HIP(hipMemsetAsync(h->pb.pairs, 0, 8 * size_t(h->pb.capacity), P->stream));
       const void* xc[3] = {P->xcur(0), P->xcur(1), P->xcur(h->dim == 3 ? 2 : 1)};
       pair_count(P->G, 64, xc, P->sb.start, P->sb.skey, h->n, h->dist, h->pb, P->stream);
       int64_t total = 0;
       GCL_HIP(hipMemcpyAsync(&total, h->pb.off + h->n, 8, hipMemcpyDeviceToHost, P->stream));
+      GCL_HIP(hipStreamSynchronize(P->stream));
+      if (fillzero) h->pb.zero_from(total, P->stream);
       pair_emit(P->G, 64, xc, P->sb.start, P->sb.skey, P->idcur(), h->n, h->dist, h->pb, P->stream);
       GCL_HIP(hipStreamSynchronize(P->stream));
       h->pairsn = total;

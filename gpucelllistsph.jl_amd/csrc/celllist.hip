@@ -275,6 +275,12 @@ void PairBuffers::set_capacity(int64_t cap, hipStream_t s) {
   }
 }
 
+void PairBuffers::zero_from(int64_t from, hipStream_t s) {
+  if (from < 0) from = 0;
+  if (from < capacity)
+    GCL_HIP(hipMemsetAsync(pairs + 2 * size_t(from), 0, sizeof(int32_t) * 2 * size_t(capacity - from), s));
+}
+
 void PairBuffers::release() {
   for (void* p : {(void*)cnt, (void*)off, (void*)scan_tmp, (void*)pairs})
     if (p) (void)hipFree(p);

@@ -103,6 +103,10 @@ struct PairBuffers {
   int64_t capacity = 0;
   void alloc(int64_t n_);
   void set_capacity(int64_t cap, hipStream_t s);  // zero-filled
+  // Zeroes the entries [from, capacity).  A search with fill-zero leaves the
+  // pairs in [0, pairsn) and zeros after them: the emit writes the front, so
+  // only the tail needs the fill.
+  void zero_from(int64_t from, hipStream_t s);
   void release();
 };
 

@@ -127,6 +127,38 @@ def test_partialupdate_overflow_keeps_true_count(G, O, S):
     assert len(G.neighborlist(g)) == g.capacity
 
 
+def test_fillzero_tail(G, O, S):
+    """update!/partialupdate! without reallocation (src/celllist.jl:124-163,
+    170-174): with fillzero = false the entries past pairsn keep the previous
+    search's contents; with fillzero = true they are (0, 0)."""
+    u = S.uniform_points(12, 30000, 2)
+    pts = tuple(0.1 + 0.8 * p for p in u)
+    kw = dict(MIN=(0.0, 0.0), MAX=(1.0, 1.0))
+    g = G.GPUCellList(pts, 0.02, 0.02, **kw)
+    o = O.CellList(pts, 0.02, 0.02, **kw)
+    cap, n0 = g.capacity, g.pairsn
+    spread = tuple(0.5 + 1.03 * (p - 0.5) for p in pts)  # fewer pairs, same grid, no reallocation
+    for upd, oupd in ((G.update_, o.update), (G.partialupdate_, o.partialupdate)):
+        g.points = pts
+        o.set_points(pts)
+        G.update_(g)
+        o.update()
+        before = G.neighborlist(g)
+        g.points = spread
+        o.set_points(spread)
+        upd(g, False)
+        oupd()
+        n1 = g.pairsn
+        assert n1 == o.pairsn and 0.6 * cap <= n1 < n0 and g.capacity == cap
+        after = G.neighborlist(g)
+        assert np.array_equal(after[n1:], before[n1:]) and np.any(after[n1:n0] != 0)
+        assert np.array_equal(as_set(after[:n1]), as_set(o.neighborlistview()))
+        upd(g, True)
+        after = G.neighborlist(g)
+        assert g.pairsn == n1 and not np.any(after[n1:])
+        assert np.array_equal(as_set(after[:n1]), as_set(o.neighborlistview()))
+
+
 def test_points_outside_min_max_clamp(G, O, S):
     pts = S.uniform_points(8, 20000, 2)
     kw = dict(MIN=(0.25, 0.2), MAX=(0.7, 0.9))
