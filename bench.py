@@ -141,6 +141,9 @@ def main():
                     help="2d: the headline line (default); 3d: the 883,376-particle 3D dam break with every "
                          "physics term, FP32-split and FP64; ensemble: only the batched-ensemble leg; celllist: "
                          "update! (binning + pair list) of a cell list over the 4 M-particle scene")
+    ap.add_argument("--cl-points", choices=("dambreak", "uniform200k"), default="dambreak",
+                    help="points of --scene celllist: the 4 M-particle dam break, or the 200,000 uniform points "
+                         "of the reference's own update!/partialupdate! benchmark")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pass2_f32split_traffic.txt"))
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="copy a buffer of this many GiB three times and exit (HBM counter calibration)")
@@ -161,6 +164,31 @@ def main():
               "(multi-GPU sharding is a design note, not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    if args.scene == "celllist" and args.cl_points == "uniform200k":
+        pts = S.config1_points()
+        d = S.CONFIG1_DIST
+        cl = G.GPUCellList(pts, (d, d), d)
+        rng = np.random.default_rng(7)
+        pts = tuple(p + rng.normal(0.0, 0.001, p.shape) for p in pts)  # one N(0, 0.001^2) jitter
+        cl.points = pts
+        res = {}
+        for name, fn in (("update!", G.update_), ("partialupdate!", G.partialupdate_)):
+            for _ in range(args.warmup):
+                fn(cl)
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                fn(cl)
+            res[name] = (time.perf_counter() - t0) / args.steps * 1e3
+        # the reference's own README benchmark of this system (BenchmarkTools medians, its GPU unstated)
+        ref = {"update!": 43.195, "partialupdate!": 42.673}
+        print(json.dumps({"metric": "cell-list update! (binning + pair list) per second", "value": 1e3 / res["update!"],
+                          "unit": "updates/s", "ms_per_update": res["update!"],
+                          "ms_per_partialupdate": res["partialupdate!"], "steps": args.steps,
+                          "reference_readme_median_ms": ref,
+                          "config": {"workload": "GPUCellList over 200,000 uniform points in [0,1)^2 (seed 12345), "
+                                                 "jittered once by N(0, 0.001^2), FP64, cell size = dist = 0.016",
+                                     "particles": cl.n, "pairs": cl.pairsn}}))
+        return
     if args.scene == "celllist":
         sc = S.config3(dx=args.dx)
         cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)

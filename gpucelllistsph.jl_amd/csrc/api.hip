@@ -170,7 +170,7 @@ struct gcl_s {
       fresh = true;
     }
     if (fillzero && !fresh) pb.zero_from(total, stream);
-    pair_emit(G, prec, xs, sb.start, sb.skey, sb.perm, n, dist, pb, stream);
+    pair_emit(G, prec, xs, sb.start, sb.skey, sb.perm, n, dist, total, pb, stream);
     pairsn = total;
     GCL_HIP(hipStreamSynchronize(stream));
   }
@@ -180,7 +180,7 @@ struct gcl_s {
     gather_positions(dim, prec, xo, xs, sb.perm, n, stream);
     int64_t total = count_pairs(xs);
     if (fillzero) pb.zero_from(total, stream);
-    pair_emit(G, prec, xs, sb.start, sb.skey, sb.perm, n, dist, pb, stream);
+    pair_emit(G, prec, xs, sb.start, sb.skey, sb.perm, n, dist, total, pb, stream);
     pairsn = total;
     GCL_HIP(hipStreamSynchronize(stream));
   }
@@ -967,7 +967,7 @@ void ensure_pairs(gcl_s* g) {
   if (total > cap) cap = ceil_div08(total);
   if (cap != g->pb.capacity) g->pb.set_capacity(cap, P->stream);
   else g->pb.zero_from(total, P->stream);
-  pair_emit(P->G, 64, xm, P->sb.start, P->sb.skey, P->idcur(), g->n, g->dist, g->pb, P->stream);
+  pair_emit(P->G, 64, xm, P->sb.start, P->sb.skey, P->idcur(), g->n, g->dist, total, g->pb, P->stream);
   GCL_HIP(hipStreamSynchronize(P->stream));
   g->pairsn = total;
   g->pairs_valid = true;
@@ -1076,7 +1076,7 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
     int64_t cap = mpairs;
     if (double(total) < double(mpairs) * 0.8) cap = ceil_div08(total);  // :109-115
     g->pb.set_capacity(cap, g->stream);
-    pair_emit(g->G, prec, g->xs, g->sb.start, g->sb.skey, g->sb.perm, n, g->dist, g->pb, g->stream);
+    pair_emit(g->G, prec, g->xs, g->sb.start, g->sb.skey, g->sb.perm, n, g->dist, total, g->pb, g->stream);
     GCL_HIP(hipStreamSynchronize(g->stream));
     g->pairsn = total;
     if (warn_flags) *warn_flags = warn;
@@ -1116,7 +1116,7 @@ int gcl_partialupdate(gcl_t h, int fillzero) {
       GCL_HIP(hipMemcpyAsync(&total, h->pb.off + h->n, 8, hipMemcpyDeviceToHost, P->stream));
       GCL_HIP(hipStreamSynchronize(P->stream));
       if (fillzero) h->pb.zero_from(total, P->stream);
-      pair_emit(P->G, 64, xc, P->sb.start, P->sb.skey, P->idcur(), h->n, h->dist, h->pb, P->stream);
+      pair_emit(P->G, 64, xc, P->sb.start, P->sb.skey, P->idcur(), h->n, h->dist, total, h->pb, P->stream);
       GCL_HIP(hipStreamSynchronize(P->stream));
       h->pairsn = total;
       h->pairs_valid = true;

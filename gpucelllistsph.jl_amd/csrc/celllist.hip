@@ -8,6 +8,9 @@
 // (:309-554): every sorted particle scans the 3^D cells around it (the rows of a
 // stencil are contiguous in cell-sorted order) and keeps partners with a larger
 // sorted index, so each unordered pair is produced once.
+#include <cstdlib>
+#include <cstring>
+
 #include "celllist.hpp"
 
 namespace gclsph {
@@ -207,6 +210,58 @@ __global__ __launch_bounds__(256) void k_pair_emit(XPtrs xsp, const int32_t* __r
   for (int64_t i = threadIdx.x & 63; i < ns && o0 + i < capacity; i += 64) out[o0 + i] = st[i];
 }
 
+// Dense lists: one wave per sorted item s.  The 64 lanes test 64 consecutive
+// candidates of a stencil row at a time; the hits go out as one contiguous run
+// at off[s] + (hits in lower lanes), so the stores coalesce.  Rows, windows and
+// candidates are visited in ascending order, as in k_pair_emit, so both kernels
+// write the same pairs to the same slots.
+template <typename T, int D>
+__global__ __launch_bounds__(256) void k_pair_emit_wave(XPtrs xsp, const int32_t* __restrict__ start,
+                                                        const int32_t* __restrict__ skey,
+                                                        const int32_t* __restrict__ id, int64_t n, CellGeom g,
+                                                        T dist2max, const int64_t* __restrict__ off,
+                                                        int32_t* __restrict__ pairs, int64_t capacity) {
+#pragma clang fp contract(off)
+  const int64_t s = int64_t(blockIdx.x) * (kBlock / 64) + __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  if (s >= n) return;  // whole wave
+  const int32_t lane = int32_t(threadIdx.x & 63);
+  const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
+                   static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
+  const T xa0 = x[0][s], xa1 = x[1][s], xa2 = D == 3 ? x[2][s] : T(0);
+  const Rows<D> r = stencil_rows<D>(g, start, skey[s]);
+  const int32_t is = id[s] + 1;
+  int64_t o = off[s];
+  int2* out = reinterpret_cast<int2*>(pairs);
+#pragma unroll
+  for (int k = 0; k < Rows<D>::N; ++k) {
+    const int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
+    for (int32_t w0 = b; w0 < r.e[k]; w0 += 64) {
+      const int32_t j = w0 + lane;
+      bool hit = false;
+      if (j < r.e[k]) {
+        const T d0 = xa0 - x[0][j];
+        const T d1 = xa1 - x[1][j];
+        T r2 = d0 * d0 + d1 * d1;
+        if (D == 3) {
+          const T d2 = xa2 - x[2][j];
+          r2 = r2 + d2 * d2;
+        }
+        hit = r2 < dist2max;
+      }
+      const uint64_t m = __ballot(hit);
+      if (hit) {
+        const int64_t slot =
+            o + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+        if (slot < capacity) {
+          const int32_t it = id[j] + 1;
+          out[slot] = is < it ? make_int2(is, it) : make_int2(it, is);
+        }
+      }
+      o += __builtin_popcountll(m);
+    }
+  }
+}
+
 template <typename T>
 void bin_sort_t(const CellGeom& g, const void* const* x, int64_t n, bool use_off64, SortBuffers& sb, const int* flag,
                 hipStream_t s, const void* const* x_alt, const int* sel) {
@@ -328,27 +383,37 @@ void pair_count(const CellGeom& g, int prec, const void* const* xs, const int32_
   exclusive_scan_i64(pb.cnt, pb.off, n, pb.scan_tmp, nullptr, s);
 }
 
+namespace {
+template <typename T, int D>
+void launch_emit(bool wave, const XPtrs& xsp, const int32_t* start, const int32_t* skey, const int32_t* id,
+                 int64_t n, const CellGeom& g, T d2, PairBuffers& pb, hipStream_t s) {
+  if (wave)
+    k_pair_emit_wave<T, D><<<grid_for(n, kBlock / 64), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off,
+                                                                       pb.pairs, pb.capacity);
+  else
+    k_pair_emit<T, D><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
+                                                     pb.capacity);
+}
+}  // namespace
+
 void pair_emit(const CellGeom& g, int prec, const void* const* xs, const int32_t* start, const int32_t* skey,
-               const int32_t* id, int64_t n, double dist, PairBuffers& pb, hipStream_t s) {
-  if (pb.capacity <= 0) return;
+               const int32_t* id, int64_t n, double dist, int64_t total, PairBuffers& pb, hipStream_t s) {
+  if (pb.capacity <= 0 || n <= 0) return;
+  bool wave = g.dim == 2 && total >= kEmitWaveMin * n;
+  if (const char* e = std::getenv("GCLSPH_PAIR_EMIT")) {
+    if (std::strcmp(e, "wave") == 0) wave = true;
+    else if (std::strcmp(e, "lane") == 0) wave = false;
+  }
   XPtrs xsp{{xs[0], xs[1], g.dim == 3 ? xs[2] : xs[1]}};
   if (prec == 32) {
     float d = float(dist);
     float d2 = d * d;
-    if (g.dim == 2)
-      k_pair_emit<float, 2><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
-                                                           pb.capacity);
-    else
-      k_pair_emit<float, 3><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
-                                                           pb.capacity);
+    if (g.dim == 2) launch_emit<float, 2>(wave, xsp, start, skey, id, n, g, d2, pb, s);
+    else launch_emit<float, 3>(wave, xsp, start, skey, id, n, g, d2, pb, s);
   } else {
     double d2 = dist * dist;
-    if (g.dim == 2)
-      k_pair_emit<double, 2><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
-                                                            pb.capacity);
-    else
-      k_pair_emit<double, 3><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
-                                                            pb.capacity);
+    if (g.dim == 2) launch_emit<double, 2>(wave, xsp, start, skey, id, n, g, d2, pb, s);
+    else launch_emit<double, 3>(wave, xsp, start, skey, id, n, g, d2, pb, s);
   }
   GCL_HIP(hipGetLastError());
 }

@@ -159,6 +159,23 @@ def test_fillzero_tail(G, O, S):
         assert np.array_equal(as_set(after[:n1]), as_set(o.neighborlistview()))
 
 
+@pytest.mark.parametrize("dim,dist", [(2, 0.05), (2, 0.015), (3, 0.124), (3, 0.06)])
+def test_pair_emit_variants_agree(G, O, S, monkeypatch, dim, dist):
+    """The one-lane-per-point and one-wave-per-point emits write the same
+    buffer (dense sets of ~80 pairs per point and sparse ones of ~7)."""
+    pts = S.uniform_points(21 + dim, 20000, dim)
+    o = O.CellList(pts, dist, dist)
+    bufs = []
+    for mode in ("lane", "wave"):
+        monkeypatch.setenv("GCLSPH_PAIR_EMIT", mode)
+        g = G.GPUCellList(pts, dist, dist)
+        G.update_(g)
+        assert g.pairsn == o.info()["pairsn"]
+        bufs.append(G.neighborlist(g))
+    assert np.array_equal(bufs[0], bufs[1])
+    assert np.array_equal(as_set(bufs[1][: o.info()["pairsn"]]), as_set(o.neighborlistview()))
+
+
 def test_points_outside_min_max_clamp(G, O, S):
     pts = S.uniform_points(8, 20000, 2)
     kw = dict(MIN=(0.25, 0.2), MAX=(0.7, 0.9))
