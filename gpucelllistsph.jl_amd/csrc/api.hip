@@ -1,6 +1,6 @@
 // Host objects and the C ABI of libgclsph (include/gclsph.h).
 //
-// gcl_s mirrors GPUCellList (src/celllist.jl:30-44) and its host driver
+// gcl_s mirrors GPUCellList (src/celllist.jl:1-15) and its host driver
 // (update!, partialupdate!, neighborlist); sph_s mirrors SPHProblem
 // (src/sphproblem.jl:62-193) and _stepsolve! (:207-345).  The SPH step keeps
 // its whole state on the device, in cell-sorted order, and runs a batch of
@@ -153,7 +153,7 @@ struct gcl_s {
     return total;
   }
 
-  // update! (src/celllist.jl:124-163) for a standalone cell list.
+  // update! (src/celllist.jl:95-134) for a standalone cell list.
   void update(bool fillzero) {
     bin_sort(G, prec, xo, n, false, sb, nullptr, stream);
     int32_t maxpoint = read_maxcell();
@@ -175,7 +175,7 @@ struct gcl_s {
     GCL_HIP(hipStreamSynchronize(stream));
   }
 
-  // partialupdate! (src/celllist.jl:170-174): current points, cells of the last update!.
+  // partialupdate! (src/celllist.jl:141-145): current points, cells of the last update!.
   void partialupdate(bool fillzero) {
     gather_positions(dim, prec, xo, xs, sb.perm, n, stream);
     int64_t total = count_pairs(xs);
@@ -987,7 +987,7 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
   return guard([&] {
     GCL_CHECK(out != nullptr, kErrArgument, "out handle is NULL");
     *out = nullptr;
-    GCL_CHECK(dim == 2 || dim == 3, kErrArgument, "wrong dimention");  // src/celllist.jl:55
+    GCL_CHECK(dim == 2 || dim == 3, kErrArgument, "wrong dimention");  // src/celllist.jl:26
     GCL_CHECK(prec == GCLSPH_F64 || prec == GCLSPH_F32, kErrArgument, "cell list precision must be F64 or F32");
     GCL_CHECK(n > 0 && n < (int64_t(1) << 31) - 1, kErrArgument, "number of points must be in [1, 2^31-2]");
     GCL_CHECK(pts != nullptr && cellsize != nullptr, kErrArgument, "points / cellsize is NULL");
@@ -998,7 +998,7 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
     g->prec = prec;
     g->n = n;
     int warn = 0;
-    // T(dist), T.(cellsize); a cell smaller than dist is raised to dist (:56-68)
+    // T(dist), T.(cellsize); a cell smaller than dist is raised to dist (:27-39)
     double distT = prec == 32 ? double(float(dist)) : dist;
     g->dist = distT;
     for (int k = 0; k < dim; ++k) {
@@ -1030,8 +1030,8 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
           mx = v > mx ? v : mx;
         }
       }
-      mn = mn - std::fabs((mn + sq) * sq);  // :71
-      double cells = std::ceil((mx - mn) / g->cs[k]);  // :73-75
+      mn = mn - std::fabs((mn + sq) * sq);  // :42
+      double cells = std::ceil((mx - mn) / g->cs[k]);  // :44-46
       GCL_CHECK(cells >= 1 && cells < 2.0e9, kErrArgument, "cell grid has no cells (or too many) along a dimension");
       g->G.grid[k] = int64_t(cells);
       g->G.off64[k] = mn;
@@ -1053,13 +1053,13 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
     g->upload_points(pts);
     g->sb.alloc(n, g->G.ncells());
     g->pb.alloc(n);
-    // cellmap! with the Float64 MIN (:84), then fillcells_naive! (:95)
+    // cellmap! with the Float64 MIN (:55), then fillcells_naive! (:66)
     bin_sort(g->G, prec, g->xo, n, true, g->sb, nullptr, g->stream);
     int32_t maxc = g->read_maxcell();
-    int64_t maxpoint = int64_t(std::ceil(double(maxc) * 1.05 + 1));  // :86-87
+    int64_t maxpoint = int64_t(std::ceil(double(maxc) * 1.05 + 1));  // :57-58
     g->mppcell = mppcell < maxpoint ? maxpoint : mppcell;
     gather_positions(dim, prec, g->xo, g->xs, g->sb.perm, n, g->stream);
-    if (mpairs == 0) {  // мaxpairs (:98-100)
+    if (mpairs == 0) {  // мaxpairs (:69-71)
       unsigned long long* d = nullptr;
       GCL_HIP(hipMalloc(&d, 8));
       GCL_HIP(hipMemsetAsync(d, 0, 8, g->stream));
@@ -1072,9 +1072,9 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
       mpairs = int64_t(v);
     }
     int64_t total = g->count_pairs(g->xs);
-    if (total > mpairs) warn |= 2;  // :106-108
+    if (total > mpairs) warn |= 2;  // :77-79
     int64_t cap = mpairs;
-    if (double(total) < double(mpairs) * 0.8) cap = ceil_div08(total);  // :109-115
+    if (double(total) < double(mpairs) * 0.8) cap = ceil_div08(total);  // :80-86
     g->pb.set_capacity(cap, g->stream);
     pair_emit(g->G, prec, g->xs, g->sb.start, g->sb.skey, g->sb.perm, n, g->dist, total, g->pb, g->stream);
     GCL_HIP(hipStreamSynchronize(g->stream));

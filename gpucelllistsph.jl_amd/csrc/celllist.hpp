@@ -6,7 +6,7 @@
 
 namespace gclsph {
 
-// Geometry of a fixed cell grid (src/celllist.jl:70-75): 1-based cell k along
+// Geometry of a fixed cell grid (src/celllist.jl:41-46): 1-based cell k along
 // an axis holds p = (x - off) * (1/cs) in (k-1, k], clamped into [1, grid].
 struct CellGeom {
   int dim;

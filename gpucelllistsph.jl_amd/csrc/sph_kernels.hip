@@ -254,7 +254,7 @@ __global__ void k_post_rebuild(SS ss) {
     ctl->rebuilt = 1;
     ctl->updaten += 1;
     long long mp = *maxcell;
-    if (mp > ctl->mppcell) ctl->mppcell = (long long)ceil(double(mp) * 1.05 + 1);  // src/celllist.jl:134-135
+    if (mp > ctl->mppcell) ctl->mppcell = (long long)ceil(double(mp) * 1.05 + 1);  // src/celllist.jl:105-106
     ctl->rebuild = 0;
     ctl->xrb_cur = write_xrb ? 0 : 1;
   } else {

@@ -60,7 +60,7 @@ class WendlandC2:
 # GPUCellList (src/celllist.jl)
 # --------------------------------------------------------------------------
 class GPUCellList:
-    """GPUCellList(points, cellsize, dist; MIN, MAX, mppcell, mpairs) — src/celllist.jl:51-117."""
+    """GPUCellList(points, cellsize, dist; MIN, MAX, mppcell, mpairs) — src/celllist.jl:22-88."""
 
     def __init__(self, points: Sequence[np.ndarray], cellsize, dist: float, MIN=None, MAX=None, mppcell: int = 0,
                  mpairs: int = 0, precision="f64"):
@@ -178,12 +178,12 @@ class GPUCellList:
 
 
 def update_(c: GPUCellList, fillzero: bool = True) -> None:
-    """update!(c, fillzero) — src/celllist.jl:124-163."""
+    """update!(c, fillzero) — src/celllist.jl:95-134."""
     check(lib().gcl_update(c._h, int(bool(fillzero))))
 
 
 def partialupdate_(c: GPUCellList, fillzero: bool = True) -> None:
-    """partialupdate!(c, fillzero) — src/celllist.jl:170-174."""
+    """partialupdate!(c, fillzero) — src/celllist.jl:141-145."""
     check(lib().gcl_partialupdate(c._h, int(bool(fillzero))))
 
 

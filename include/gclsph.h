@@ -50,7 +50,7 @@ const char* gclsph_version(void);
 /* ---------------------------------------------------------------- GPUCellList */
 
 /* GPUCellList(points, cellsize, dist; MIN, MAX, mppcell, mpairs)
- * — src/celllist.jl:51-117.  pts: dim host arrays of n coordinates.
+ * — src/celllist.jl:22-88.  pts: dim host arrays of n coordinates.
  * min/max: dim values or NULL (then the point extrema are used).
  * warn_flags (nullable): bit0 = a cell size was raised to dist (:61-68),
  * bit1 = the first pair list did not fit mpairs (:106-108). */
@@ -59,10 +59,10 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
                int* warn_flags, gcl_t* out);
 int gcl_destroy(gcl_t h);
 
-/* update!(c, fillzero) — src/celllist.jl:124-163: re-bin the current points,
+/* update!(c, fillzero) — src/celllist.jl:95-134: re-bin the current points,
  * regrow the pair buffer by the reference's rule and rebuild the pair list. */
 int gcl_update(gcl_t h, int fillzero);
-/* partialupdate!(c, fillzero) — src/celllist.jl:170-174: pair list of the
+/* partialupdate!(c, fillzero) — src/celllist.jl:141-145: pair list of the
  * current points over the cells of the last update!; no regrowth. */
 int gcl_partialupdate(gcl_t h, int fillzero);
 
@@ -71,10 +71,10 @@ int gcl_partialupdate(gcl_t h, int fillzero);
 int gcl_set_points(gcl_t h, const double* const* pts);
 int gcl_get_points(gcl_t h, double* const* out);
 
-/* c.pairsn and length(c.pairs) — src/celllist.jl:41-43. */
+/* c.pairsn and length(c.pairs) — src/celllist.jl:13-14. */
 int gcl_pairsn(gcl_t h, int64_t* out);
 int gcl_capacity(gcl_t h, int64_t* out);
-/* neighborlist(c) — src/celllist.jl:181-183: the whole pair buffer,
+/* neighborlist(c) — src/celllist.jl:152-154: the whole pair buffer,
  * 2*min(cap, capacity) int32 values (i, j), 1-based, i < j, zero padded.
  * neighborlistview(c) (:190-192) is the first pairsn of them; it fails with
  * GCLSPH_ESTATE when pairsn > capacity, as the reference's view would. */

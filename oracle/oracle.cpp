@@ -71,7 +71,7 @@ struct CellList {
 };
 
 // kernel_cellmap! (src/gpukernels_cls.jl:29-60); hinv = 1/h on the host in T
-// (:21).  `use_off64` reproduces the constructor call (src/celllist.jl:84),
+// (:21).  `use_off64` reproduces the constructor call (src/celllist.jl:55),
 // which passes the Float64 MIN before it is stored in T (matters for T=Float32).
 template <typename T>
 void cellmap(CellList<T>& c, bool use_off64) {
@@ -223,7 +223,7 @@ void resize_pairs(CellList<T>& c, int64_t cap) {
   c.pairs.assign(size_t(2 * cap), 0);
 }
 
-// GPUCellList constructor (src/celllist.jl:51-117)
+// GPUCellList constructor (src/celllist.jl:22-88)
 template <typename T>
 int celllist_create(CellList<T>& c, int D, int64_t n, const double* const* pts, const double* cellsize,
                     double dist, const double* MINin, const double* MAXin, int64_t mppcell, int64_t mpairs,
@@ -240,7 +240,7 @@ int celllist_create(CellList<T>& c, int D, int64_t n, const double* const* pts, 
   *warn_flags = 0;
   for (int k = 0; k < D; ++k) {
     c.cs[k] = T(cellsize[k]);
-    if (c.cs[k] < c.dist) {  // :61-68
+    if (c.cs[k] < c.dist) {  // :32-39
       c.cs[k] = c.dist;
       *warn_flags |= 1;
     }
@@ -251,21 +251,21 @@ int celllist_create(CellList<T>& c, int D, int64_t n, const double* const* pts, 
     double mn, mx;
     if (MINin) mn = MINin[k];
     else mn = double(*std::min_element(c.x[k].begin(), c.x[k].end()));
-    // MIN = @. MIN - abs((MIN + sqrt(eps())) * sqrt(eps()))  (:71)
+    // MIN = @. MIN - abs((MIN + sqrt(eps())) * sqrt(eps()))  (:42)
     mn = mn - std::fabs((mn + sq) * sq);
     if (MAXin) mx = MAXin[k];
     else mx = double(*std::max_element(c.x[k].begin(), c.x[k].end()));
     c.off64[k] = mn;
     c.off[k] = T(mn);
     range[k] = mx - mn;
-    double cells = std::ceil(range[k] / double(c.cs[k]));  // :75
+    double cells = std::ceil(range[k] / double(c.cs[k]));  // :46
     if (!(cells >= 1)) return fail("cell grid has no cells along a dimension");
     c.grid[k] = int64_t(cells);
   }
   c.cellpnum.assign(size_t(c.ncells()), 0);
   cellmap(c, true);
   int32_t mx = *std::max_element(c.cellpnum.begin(), c.cellpnum.end());
-  int64_t maxpoint = int64_t(std::ceil(double(mx) * 1.05 + 1));  // :86
+  int64_t maxpoint = int64_t(std::ceil(double(mx) * 1.05 + 1));  // :57
   if (mppcell < maxpoint) mppcell = maxpoint;
   c.mppcell = mppcell;
   c.celllist.assign(size_t(c.mppcell * c.ncells()), 0);
@@ -273,8 +273,8 @@ int celllist_create(CellList<T>& c, int D, int64_t n, const double* const* pts, 
   if (mpairs == 0) mpairs = maxpairs(c);
   resize_pairs(c, mpairs);
   int64_t total = neib_search(c);
-  if (total > mpairs) *warn_flags |= 2;  // :106-108
-  if (double(total) < double(mpairs) * 0.8) {  // :109-115
+  if (total > mpairs) *warn_flags |= 2;  // :77-79
+  if (double(total) < double(mpairs) * 0.8) {  // :80-86
     int64_t newn = int64_t(std::ceil(double(total) / 0.8));
     std::vector<int32_t> keep(c.pairs.begin(), c.pairs.begin() + 2 * newn);
     c.capacity = newn;
@@ -284,7 +284,7 @@ int celllist_create(CellList<T>& c, int D, int64_t n, const double* const* pts, 
   return 0;
 }
 
-// update! (src/celllist.jl:124-163)
+// update! (src/celllist.jl:95-134)
 template <typename T>
 void celllist_update(CellList<T>& c, bool fillzero) {
   cellmap(c, false);
@@ -308,7 +308,7 @@ void celllist_update(CellList<T>& c, bool fillzero) {
   }
 }
 
-// partialupdate! (src/celllist.jl:170-174)
+// partialupdate! (src/celllist.jl:141-145)
 template <typename T>
 void celllist_partialupdate(CellList<T>& c, bool fillzero) {
   if (fillzero) std::fill(c.pairs.begin(), c.pairs.end(), 0);

@@ -128,8 +128,8 @@ def test_partialupdate_overflow_keeps_true_count(G, O, S):
 
 
 def test_fillzero_tail(G, O, S):
-    """update!/partialupdate! without reallocation (src/celllist.jl:124-163,
-    170-174): with fillzero = false the entries past pairsn keep the previous
+    """update!/partialupdate! without reallocation (src/celllist.jl:95-134,
+    141-145): with fillzero = false the entries past pairsn keep the previous
     search's contents; with fillzero = true they are (0, 0)."""
     u = S.uniform_points(12, 30000, 2)
     pts = tuple(0.1 + 0.8 * p for p in u)
@@ -196,7 +196,7 @@ def test_cellsize_below_dist_warns(G, S):
 def test_degenerate_inputs(G, S):
     g = G.GPUCellList((np.array([0.5]), np.array([0.5])), 0.1, 0.1, MIN=(0.0, 0.0), MAX=(1.0, 1.0))
     assert g.pairsn == 0 and len(G.neighborlistview(g)) == 0
-    # identical points: the lowered MIN (src/celllist.jl:71) still gives one cell; all pairs at r = 0
+    # identical points: the lowered MIN (src/celllist.jl:42) still gives one cell; all pairs at r = 0
     g = G.GPUCellList((np.full(10, 0.3), np.full(10, 0.3)), 0.1, 0.1)
     assert g.grid == (1, 1) and g.pairsn == 45
     with pytest.raises(ValueError, match="no cells"):

@@ -176,12 +176,12 @@ def test_pair_capacity_rules(O, S):
     dist = 0.05
     full = O.CellList(pts, dist, dist)
     total = full.pairsn
-    assert full.info()["capacity"] == math.ceil(total / 0.8)  # shrink rule, src/celllist.jl:109-115
+    assert full.info()["capacity"] == math.ceil(total / 0.8)  # shrink rule, src/celllist.jl:80-86
     small = O.CellList(pts, dist, dist, mpairs=100)
     assert small.warn & 2 and small.pairsn == total and small.info()["capacity"] == 100
     with pytest.raises(O.OracleError):
         small.neighborlistview()
-    small.update()  # regrow to ceil(pairsn / 0.8) (src/celllist.jl:146-148)
+    small.update()  # regrow to ceil(pairsn / 0.8) (src/celllist.jl:117-119)
     assert small.info()["capacity"] == math.ceil(total / 0.8)
     assert set(map(tuple, small.neighborlistview().tolist())) == set(map(tuple, full.neighborlistview().tolist()))
 
@@ -201,7 +201,7 @@ def test_float32_celllist(O, S):
     cl = O.CellList(pts, 0.06, 0.06, prec=32)
     p32 = [p.astype(np.float32) for p in pts]
     inf = cl.info()
-    # constructor binning uses the Float64 offset with Float32 points (src/celllist.jl:84)
+    # constructor binning uses the Float64 offset with Float32 points (src/celllist.jl:55)
     cells = []
     for k in range(2):
         hinv = np.float32(1) / np.float32(inf["cs"][k])
