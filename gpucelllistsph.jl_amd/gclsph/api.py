@@ -230,7 +230,10 @@ class SPHProblem:
             vv = [np.ascontiguousarray(a, dtype=np.float64) for a in v]
             if len(vv) != system.dim or any(len(a) != N for a in vv):
                 raise ValueError(f"length(v) ({[len(a) for a in vv]}) != N ({N})")
-        if sphkernel is not None and getattr(sphkernel, "dim", system.dim) != system.dim:
+        if sphkernel is not None and not isinstance(sphkernel, WendlandC2):
+            # the kernels evaluate SPHKernels.jl's WendlandC2 only (the one the reference's examples use)
+            raise NotImplementedError(f"SPH kernel {sphkernel!r}: only WendlandC2 is implemented")
+        if sphkernel is not None and sphkernel.dim != system.dim:
             raise ValueError("SPH kernel dimension differs from the system's")
         par = {"dx": dx, "h": h, "H": H, "rho0": rho0, "m0": m0, "dt": dt, "alpha": alpha, "c0": c0, "gamma": gamma,
                "delta": delta, "CFL": CFL, "g": g, "s": s, "nu": 0.0, "dpc_l0": 0.0, "dpc_pmin": 1.0,

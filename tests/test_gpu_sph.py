@@ -233,6 +233,26 @@ def test_ptype_getter_and_stepsolve_zero(G, O, S):
     assert rel_err(G.get_sumw(prob), ref.get("sumW")) < 1e-12
 
 
+def test_unimplemented_sph_kernel_is_refused(G, S):
+    """Only WendlandC2 is evaluated; another kernel object is an error, not
+    a silent substitution."""
+    sc = S.config0()
+    cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+    p = sc.params
+
+    class CubicSpline:
+        dim = 2
+
+    args = (cl, p.dx, p.h, p.H, CubicSpline(), sc.rho, sc.ptype, p.rho0, p.m0, p.dt, p.alpha, p.c0, p.gamma,
+            p.delta, p.CFL)
+    with pytest.raises(NotImplementedError, match="WendlandC2"):
+        G.SPHProblem(*args)
+    with pytest.raises(ValueError, match="dimension"):
+        G.SPHProblem(*args[:4], G.WendlandC2(np.float64, 3), *args[5:])
+    prob = G.SPHProblem(*args[:4], G.WendlandC2(np.float64, 2), *args[5:])
+    assert prob.dim == 2
+
+
 def test_full_size_f32split_vs_f64_1000_steps(G, S):
     """The 4,047,991-particle 2D dam break after 1000 adaptive steps.
 
