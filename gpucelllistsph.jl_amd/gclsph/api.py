@@ -412,6 +412,7 @@ def timesolve_(prob: SPHProblem, batch: int = 10, timeframe: float = 1.0, writet
                 pvd = paraview_collection(os.path.join(path, "OUTPUT_PVC"))
             writer.submit(prob, os.path.join(path, f"OUTPUT_{i:05d}"), vtkvars, pvd=pvd, writetime=True)
         diag = None
+        prog = _Progress("Calculating...:") if verbose else None
         while prob.etime <= timeframe:
             diag = stepsolve_(prob, batch, timestepping=timestepping, timelims=timelims)
             if writetime > 0 and nt < prob.etime:
@@ -419,12 +420,45 @@ def timesolve_(prob: SPHProblem, batch: int = 10, timeframe: float = 1.0, writet
                 if path is not None:
                     writer.submit(prob, os.path.join(path, f"OUTPUT_{i:05d}"), vtkvars, pvd=pvd, writetime=True)
             i += 1
+            if prog is not None:
+                prog.next(iter=i, time=prob.etime, Δt=prob.dt, updn=diag[0], dxpncu=diag[1])
     finally:
         if writer is not None:
             writer.close()
     if write and pvc:
         vtk_save(pvd)
+    if prog is not None:
+        prog.finish()
     return diag
+
+
+class _Progress:
+    """Per-batch progress of timesolve_, the values ProgressMeter shows in the
+    reference (src/sphproblem.jl:456,498,510: `ProgressUnknown` with
+    showvalues iter, time, Δt, updn, dxpncu, then `finish!`), one line per
+    batch on stderr with the batch rate."""
+
+    def __init__(self, desc: str, stream=None):
+        import sys
+        import time
+        self.desc, self.stream, self.count = desc, stream or sys.stderr, 0
+        self.t0 = time.perf_counter()
+
+    def _rate(self):
+        import time
+        el = time.perf_counter() - self.t0
+        return el, (self.count / el if el > 0 else 0.0)
+
+    def next(self, **values):
+        self.count += 1
+        el, rate = self._rate()
+        vals = "  ".join(f"{k}: {v}" for k, v in values.items())
+        print(f"{self.desc} {self.count} \t Time: {el:.2f} s ({rate:.2f} it/s)  {vals}", file=self.stream,
+              flush=True)
+
+    def finish(self):
+        el, rate = self._rate()
+        print(f"{self.desc} {self.count} \t Time: {el:.2f} s ({rate:.2f} it/s) done", file=self.stream, flush=True)
 
 
 def get_points(prob: SPHProblem):

@@ -35,7 +35,7 @@ extern "C" {
 /* Precision of a cell list (its point type T) or of an SPH problem. */
 #define GCLSPH_F64 64        /* Float64 everywhere (the reference's runs) */
 #define GCLSPH_F32 32        /* GPUCellList{Float32} (cell list only) */
-#define GCLSPH_F32SPLIT 33   /* SPH: FP32 pair sums (Kahan), density as rho-rho0,
+#define GCLSPH_F32SPLIT 33   /* SPH: FP32 pair terms summed in FP64, density as rho-rho0,
                                 FP64 positions and FP64 dt reduction */
 
 typedef struct gcl_s* gcl_t;
@@ -52,8 +52,8 @@ const char* gclsph_version(void);
 /* GPUCellList(points, cellsize, dist; MIN, MAX, mppcell, mpairs)
  * — src/celllist.jl:22-88.  pts: dim host arrays of n coordinates.
  * min/max: dim values or NULL (then the point extrema are used).
- * warn_flags (nullable): bit0 = a cell size was raised to dist (:61-68),
- * bit1 = the first pair list did not fit mpairs (:106-108). */
+ * warn_flags (nullable): bit0 = a cell size was raised to dist (:32-39),
+ * bit1 = the first pair list did not fit mpairs (:77-79). */
 int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const double* cellsize, double dist,
                const double* min_or_null, const double* max_or_null, int64_t mppcell, int64_t mpairs,
                int* warn_flags, gcl_t* out);
@@ -76,7 +76,7 @@ int gcl_pairsn(gcl_t h, int64_t* out);
 int gcl_capacity(gcl_t h, int64_t* out);
 /* neighborlist(c) — src/celllist.jl:152-154: the whole pair buffer,
  * 2*min(cap, capacity) int32 values (i, j), 1-based, i < j, zero padded.
- * neighborlistview(c) (:190-192) is the first pairsn of them; it fails with
+ * neighborlistview(c) (:161-163) is the first pairsn of them; it fails with
  * GCLSPH_ESTATE when pairsn > capacity, as the reference's view would. */
 int gcl_get_pairs(gcl_t h, int32_t* ij, int64_t cap, int64_t* written);
 

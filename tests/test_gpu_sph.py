@@ -202,6 +202,28 @@ def test_timesolve_batches(G, O, S):
     assert np.abs(np.array(G.get_points(prob)) - ref.get("x")).max() < 1e-12 * sc.params.dx
 
 
+def test_timesolve_progress_lines(G, O, S, capsys):
+    """timesolve!'s per-batch progress (src/sphproblem.jl:456,498,510): one
+    line per batch with iter, time, Δt, updn and dxpncu of that batch, then a
+    closing line; the values are those stepsolve! returned."""
+    sc = S.dam_break_2d(0.05, 0.5, 1.0, 2.0, 1.5)
+    cl, oc, prob, ref = _pair(G, O, sc)
+    G.timesolve_(prob, batch=7, timeframe=1e-4, timestepping=True, verbose=True)
+    err = capsys.readouterr().err.strip().splitlines()
+    lines = [l for l in err if "iter:" in l]
+    i = 0
+    while ref.etime <= 1e-4:
+        updn, dxpncu = ref.stepsolve(7, True)
+        i += 1
+        fields = dict(kv.split(": ") for kv in lines[i - 1].split("  ") if kv.count(": ") == 1)
+        assert int(fields["iter"]) == i
+        assert float(fields["time"]) == pytest.approx(ref.etime, rel=1e-12)
+        assert float(fields["Δt"]) == pytest.approx(ref.dt, rel=1e-9)
+        assert int(fields["updn"]) == updn
+        assert float(fields["dxpncu"]) == pytest.approx(dxpncu, rel=1e-9, abs=1e-300)
+    assert len(lines) == i and err[-1].endswith("done")
+
+
 def test_gamma_other_than_7(G, O, S):
     sc = S.dam_break_2d(0.05, 0.5, 1.0, 2.0, 1.5)
     sc.params.gamma = 5
