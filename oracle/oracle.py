@@ -20,8 +20,12 @@ LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
 _lib = None
 
 
+OMP_PATH = os.path.join(HERE, "_build", "libcpuomp.so")
+_omp = None
+
+
 def build(force: bool = False) -> str:
-    if force or not os.path.exists(LIB_PATH):
+    if force or not os.path.exists(LIB_PATH) or not os.path.exists(OMP_PATH):
         subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
     return LIB_PATH
 
@@ -245,3 +249,91 @@ def dpckernlim(dim: int) -> float:
 
 def eos(rho: float, gamma: int, rho0: float, c0: float, ptype: int) -> float:
     return lib().orc_eos(rho, gamma, rho0, c0, ptype)
+
+
+def omp_lib():
+    """The multi-core CPU step (oracle/cpu_omp.cpp): the CPU baseline."""
+    global _omp
+    if _omp is None:
+        if not os.path.exists(OMP_PATH):
+            build()
+        L = C.CDLL(OMP_PATH)
+        P = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        L.orc_last_error.restype = C.c_char_p
+        L.omp_sph_create.restype = P
+        L.omp_sph_create.argtypes = [C.c_int, C.c_int64, C.POINTER(dp), dp, C.c_double, dp, dp,
+                                     C.POINTER(C.c_int32), C.POINTER(dp)]
+        L.omp_sph_destroy.argtypes = [P]
+        L.omp_sph_stepsolve.argtypes = [P, C.c_int64, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int64), dp]
+        L.omp_sph_get_field.argtypes = [P, C.c_int, dp]
+        L.omp_sph_get_scalar.restype = C.c_double
+        L.omp_sph_get_scalar.argtypes = [P, C.c_int]
+        L.omp_sph_threads.argtypes = [P]
+        L.omp_sph_pairs.restype = C.c_int64
+        L.omp_sph_pairs.argtypes = [P]
+        _omp = L
+    return _omp
+
+
+class OmpSPHProblem:
+    """The SPH step on all host cores (OpenMP per-particle gather over a CSR
+    neighbour list, FP64): same interface subset as SPHProblem.  The cell
+    grid follows the reference's GPUCellList constructor over the points."""
+
+    def __init__(self, points, cellsize, dist: float, params, rho, ptype, v=None):
+        L = omp_lib()
+        self.dim = len(points)
+        pts = [np.ascontiguousarray(p, dtype=np.float64) for p in points]
+        self.n = pts[0].shape[0]
+        cs = np.ascontiguousarray(np.broadcast_to(np.asarray(cellsize, dtype=np.float64), (self.dim,)))
+        par = np.ascontiguousarray(params.as_array())
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        pt = np.ascontiguousarray(ptype, dtype=np.int32)
+        vv = None if v is None else [np.ascontiguousarray(a, dtype=np.float64) for a in v]
+        h = L.omp_sph_create(self.dim, self.n, _ptr_array(pts), _dptr(cs), float(dist), _dptr(par), _dptr(rho),
+                             pt.ctypes.data_as(C.POINTER(C.c_int32)), _ptr_array(vv) if vv is not None else None)
+        if not h:
+            raise OracleError(L.orc_last_error().decode())
+        self.h = h
+        self.params = params
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            omp_lib().omp_sph_destroy(self.h)
+            self.h = None
+
+    @property
+    def threads(self) -> int:
+        return omp_lib().omp_sph_threads(self.h)
+
+    @property
+    def pairs(self) -> int:
+        return omp_lib().omp_sph_pairs(self.h)
+
+    def stepsolve(self, n: int = 1, timestepping: bool = False, timelims: Optional[Tuple[float, float]] = None):
+        p = self.params
+        if timelims is None:
+            timelims = (np.sqrt(np.finfo(np.float64).eps), p.CFL * p.H / (3 * p.c0))
+        upd = C.c_int64(0)
+        mx = C.c_double(0)
+        rc = omp_lib().omp_sph_stepsolve(self.h, int(n), int(bool(timestepping)), float(timelims[0]),
+                                         float(timelims[1]), C.byref(upd), C.byref(mx))
+        if rc:
+            raise OracleError(omp_lib().orc_last_error().decode())
+        return upd.value, mx.value
+
+    def get(self, name: str) -> np.ndarray:
+        code, vec = FIELDS[name]
+        out = np.empty(self.n * (self.dim if vec else 1))
+        if omp_lib().omp_sph_get_field(self.h, code, _dptr(out)):
+            raise OracleError(omp_lib().orc_last_error().decode())
+        return out.reshape(self.dim, self.n) if vec else out
+
+    @property
+    def dt(self) -> float:
+        return omp_lib().omp_sph_get_scalar(self.h, 0)
+
+    @property
+    def etime(self) -> float:
+        return omp_lib().omp_sph_get_scalar(self.h, 1)

@@ -89,3 +89,52 @@ def test_batch_argument_errors(G, S):
     with pytest.raises(ValueError, match="one timelims pair"):
         G.batch_stepsolve_(batch, 1, timelims=[(1e-8, 1e-5), (1e-8, 1e-5)])
     del batch
+
+
+def _oracle_pair(O, sc):
+    oc = O.CellList(sc.points, sc.cellsize, sc.dist)
+    return O.SPHProblem(oc, sc.params, sc.rho, sc.ptype, v=sc.v)
+
+
+def test_config4_ensemble_64_scenes(G, O, S):
+    """The 64-scene seeded ensemble (1.5 k-120 k particles per scene) as one
+    FP32-split batch, 12 adaptive steps in two calls: every scene ends
+    bit-identical to the same scene stepped alone (tile maps, descriptor array
+    and captured step of the full-size batch).  An FP64 batch of the same
+    scenes matches the CPU oracle on the smallest and the largest scene, and
+    the FP32-split batch stays within the FP32-split bounds of it."""
+    scs = S.config4()
+    assert len(scs) == 64
+    sizes = [sc.n for sc in scs]
+    lo, hi = int(np.argmin(sizes)), int(np.argmax(sizes))
+    assert sizes[lo] < 5000 and sizes[hi] > 100000
+    steps = (10, 2)
+    probs = [make(G, sc, "f32split") for sc in scs]
+    batch = G.SPHBatch(probs)
+    got = [G.batch_stepsolve_(batch, n, timestepping=True) for n in steps]
+    for i, sc in enumerate(scs):
+        p = make(G, sc, "f32split")
+        res = [G.stepsolve_(p, n, timestepping=True) for n in steps]
+        for k, (upd, mx) in enumerate(got):
+            assert (int(upd[i]), float(mx[i])) == res[k], i
+        ref, f = fields(G, p), fields(G, probs[i])
+        for name in ref:
+            assert np.array_equal(f[name], ref[name]), (i, name)
+        del p
+    p64 = [make(G, sc, "f64") for sc in scs]
+    b64 = G.SPHBatch(p64)
+    for n in steps:
+        G.batch_stepsolve_(b64, n, timestepping=True)
+    for i in (lo, hi):
+        sc = scs[i]
+        ref = _oracle_pair(O, sc)
+        for n in steps:
+            ref.stepsolve(n, True)
+        dx = sc.params.dx
+        assert p64[i].etime == pytest.approx(ref.etime, rel=1e-12)
+        x64 = np.array(G.get_points(p64[i]))
+        assert np.abs(x64 - ref.get("x")).max() < 1e-12 * dx, i
+        assert np.abs(G.get_density(p64[i]) / ref.get("rho") - 1).max() < 1e-12, i
+        x32 = np.array(G.get_points(probs[i]))
+        assert np.abs(x32 - x64).max() < 1e-5 * dx, i
+        assert np.abs(G.get_density(probs[i]) / G.get_density(p64[i]) - 1).max() < 1e-6, i
