@@ -96,6 +96,8 @@ int64_t ceil_div08(int64_t v) { return int64_t(std::ceil(double(v) / 0.8)); }
 
 }  // namespace
 
+void gclsph::set_last_error(const std::string& m) { g_last_error = m; }
+
 struct sph_s;
 
 // --------------------------------------------------------------------- gcl_s

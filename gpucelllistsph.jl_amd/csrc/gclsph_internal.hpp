@@ -36,6 +36,9 @@ enum : int {
     if (!(cond)) throw ::gclsph::Error((code), std::string(msg));  \
   } while (0)
 
+// Message returned by gclsph_last_error() on this thread (api.hip).
+void set_last_error(const std::string& m);
+
 constexpr int kBlock = 256;
 
 inline unsigned grid_for(int64_t n, int block = kBlock) {

@@ -84,6 +84,8 @@ def lib():
         "sph_batch_create": [C.POINTER(P), C.c_int32, C.POINTER(P)],
         "sph_batch_destroy": [P],
         "sph_batch_stepsolve": [P, C.c_int64, C.c_int, dp, dp, i64p, dp],
+        "gcl_neighborlist_noalloc": [C.c_int, C.c_int64, C.POINTER(C.c_void_p), dp, C.c_double, dp, dp, C.c_void_p,
+                                     C.c_int64, i64p, C.c_void_p],
     }
     for name, args in sigs.items():
         f = getattr(L, name)
@@ -103,4 +105,4 @@ EXPORTED = ("gclsph_last_error", "gclsph_version", "gcl_create", "gcl_destroy", 
             "gcl_set_points", "gcl_get_points", "gcl_pairsn", "gcl_capacity", "gcl_get_pairs", "gcl_info",
             "gcl_get_cellpnum", "gcl_get_pcell", "gcl_maxpairs", "sph_create", "sph_destroy", "sph_set_param",
             "sph_get_param", "sph_stepsolve", "sph_get_field", "sph_set_timing", "sph_get_timing", "sph_batch_create",
-            "sph_batch_destroy", "sph_batch_stepsolve")
+            "sph_batch_destroy", "sph_batch_stepsolve", "gcl_neighborlist_noalloc")

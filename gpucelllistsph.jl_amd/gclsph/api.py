@@ -197,6 +197,32 @@ def neighborlist(c: GPUCellList) -> np.ndarray:
     return out
 
 
+NOALLOC_MAX_POINTS = 524288
+NOALLOC_MAX_CELLS = 1048576
+
+
+def neighborlist_noalloc(pts_dev: Sequence[int], n: int, cellsize, dist: float, MIN, MAX, pairs_dev: int,
+                         capacity: int, stream: int = 0) -> int:
+    """The pair list of GPUCellList(points, cellsize, dist; MIN, MAX) +
+    neighborlistview without any device allocation (gcl_neighborlist_noalloc;
+    no counterpart in the reference).  pts_dev: D device addresses of n
+    Float64 coordinates; pairs_dev: device address of 2*capacity int32, which
+    receives the first min(pairsn, capacity) pairs (1-based, i < j) in the
+    order gcl_get_pairs gives.  Returns pairsn.  n <= NOALLOC_MAX_POINTS and
+    at most NOALLOC_MAX_CELLS cells; calls on one device share a fixed
+    scratch array."""
+    D = len(pts_dev)
+    ptrs = (C.c_void_p * D)(*[C.c_void_p(int(p)) for p in pts_dev])
+    cs = np.ascontiguousarray(np.broadcast_to(np.asarray(cellsize, dtype=np.float64), (D,)))
+    mn = np.ascontiguousarray(np.broadcast_to(np.asarray(MIN, dtype=np.float64), (D,)))
+    mx = np.ascontiguousarray(np.broadcast_to(np.asarray(MAX, dtype=np.float64), (D,)))
+    out = C.c_int64(0)
+    check(lib().gcl_neighborlist_noalloc(D, int(n), ptrs, _dp(cs), float(dist), _dp(mn), _dp(mx),
+                                         C.c_void_p(int(pairs_dev)), int(capacity), C.byref(out),
+                                         C.c_void_p(int(stream)) if stream else None))
+    return int(out.value)
+
+
 def neighborlistview(c: GPUCellList) -> np.ndarray:
     """neighborlistview(c) — view(c.pairs, 1:c.pairsn); BoundsError if pairsn > capacity."""
     n = c.pairsn

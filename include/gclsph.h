@@ -97,6 +97,31 @@ int gcl_get_pcell(gcl_t h, int32_t* out);
 /* мaxpairs(c.cellpnum) — src/gpukernels_cls.jl:109-286. */
 int gcl_maxpairs(gcl_t h, int64_t* out);
 
+/* Neighbour list without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code that must not
+ * call hipMalloc: stream-captured sequences, tight serving loops, harnesses
+ * that forbid it).  It computes what GPUCellList(points, cellsize, dist;
+ * MIN, MAX) followed by neighborlistview returns (src/celllist.jl:22-88,
+ * :161-163): the same pairs in the same order as gcl_create + gcl_get_pairs
+ * on the same Float64 points, 1-based (i, j) with i < j, with the cell grid
+ * of the given MIN / MAX (host arrays of dim values; points outside clamp
+ * into the edge cells).  pts_dev: dim DEVICE pointers to n Float64
+ * coordinates.  pairs_dev: DEVICE buffer of 2*capacity int32; the first
+ * min(pairsn, capacity) pairs are written, entries after them are left as
+ * they were.  pairsn (host): the total number of pairs.  stream: the
+ * hipStream_t to run on (NULL: the null stream); the call returns when the
+ * list is complete.  Intermediates live in a fixed device array of the
+ * library (38 MB per device, held until the process ends), which bounds
+ * n <= GCLSPH_NOALLOC_MAX_POINTS and the cell grid <=
+ * GCLSPH_NOALLOC_MAX_CELLS cells (GCLSPH_EARG beyond).  Calls on one device
+ * share that array: host threads are serialised, but calls that overlap on
+ * one device in any other way race. */
+#define GCLSPH_NOALLOC_MAX_POINTS 524288
+#define GCLSPH_NOALLOC_MAX_CELLS 1048576
+int gcl_neighborlist_noalloc(int dim, int64_t n, const double* const* pts_dev, const double* cellsize, double dist,
+                             const double* min, const double* max, int32_t* pairs_dev, int64_t capacity,
+                             int64_t* pairsn, void* stream);
+
 /* ---------------------------------------------------------------- SPHProblem */
 
 /* Parameter indices for sph_set_param / sph_get_param: the mutable fields of

@@ -64,3 +64,14 @@ def test_argument_errors_before_device(G):
         G.GPUCellList((np.zeros(3), np.arange(4.0)), (0.5, 0.5), 0.5)
     with pytest.raises(ValueError, match="wrong dimention"):
         G.GPUCellList((np.zeros(3),), (0.5,), 0.5)
+
+
+def test_noalloc_rejects_oversized_inputs_before_touching_a_device(G):
+    """gcl_neighborlist_noalloc checks its fixed maxima on the host first, so
+    these errors need no GPU."""
+    with pytest.raises(ValueError, match="NOALLOC_MAX_POINTS"):
+        G.neighborlist_noalloc([1, 2], G.api.NOALLOC_MAX_POINTS + 1, 0.1, 0.1, (0, 0), (1, 1), 3, 8)
+    with pytest.raises(ValueError, match="NOALLOC_MAX_CELLS"):
+        G.neighborlist_noalloc([1, 2], 1000, 1e-4, 1e-4, (0, 0), (1, 1), 3, 8)
+    with pytest.raises(ValueError, match="dimention"):
+        G.neighborlist_noalloc([1], 10, 0.1, 0.1, (0,), (1,), 3, 8)
