@@ -31,10 +31,11 @@ import gclsph as G  # noqa: E402
 from gclsph import scenes as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
-# The first call of more than 4 steps on a problem or batch times the 4 pass
-# variants on its first 4 steps; a warmup call of at least 5 steps keeps that
-# tuning out of the timed call.
-TUNE_STEPS = 5
+# The first call of more than 6 steps on a problem or batch times the pass
+# variants on its first 6 steps (4 code-generation variants, then 2 more block
+# sizes); a warmup call of at least 7 steps keeps that tuning out of the timed
+# call.
+TUNE_STEPS = 7
 
 
 def log(msg: str) -> None:
@@ -71,7 +72,8 @@ def run_gpu(scene, precision: str, steps: int, warmup: int, stage_split: bool = 
     t0 = time.perf_counter()
     upd, _ = G.stepsolve_(prob, steps, timestepping=True, timelims=tl)
     t1 = time.perf_counter()
-    out = {"seconds": t1 - t0, "updaten": upd, "dt": prob.dt, "warmup_steps": w + 2}
+    out = {"seconds": t1 - t0, "updaten": upd, "dt": prob.dt, "warmup_steps": w + 2,
+           "pass_variants": prob.pass_variants()}
     if stage_split:
         prob.set_timing(True)
         t2 = time.perf_counter()
@@ -151,8 +153,9 @@ def run_ensemble(steps: int, warmup: int):
     for p in probs:
         G.stepsolve_(p, steps, timestepping=True)
     t1 = time.perf_counter() - t0
+    tuned = batch.pass_variants()
     del batch
-    return {"scenes": len(scs), "particles": total, "min_particles": min(sc.n for sc in scs),
+    return {"scenes": len(scs), "pass_variants": tuned, "particles": total, "min_particles": min(sc.n for sc in scs),
             "max_particles": max(sc.n for sc in scs), "steps": steps,
             "batched_particle_steps_per_s": total * steps / tb, "batched_ms_per_step": tb / steps * 1e3,
             "one_by_one_particle_steps_per_s": total * steps / t1, "one_by_one_ms_per_step": t1 / steps * 1e3,
@@ -335,6 +338,7 @@ def main():
             "rebuilds_in_timed_steps": res32["updaten"],
             "precision": "FP32-split",
             "timed_path": "captured step graph, tuned pass variants",
+            "pass_variants": res32["pass_variants"],
         },
         # ms per timed step (a stage that does not run every step, such as CSPM
         # every cspmn steps under "corrections", is spread over all steps) and
@@ -360,6 +364,7 @@ def main():
     if res64 is not None:
         v64 = n * args.steps / res64["seconds"]
         out["f64"] = {"value": v64, "ms_per_step": res64["seconds"] / args.steps * 1e3,
+                      "pass_variants": res64["pass_variants"],
                       "stage_ms_per_step": {k: v[0] / args.steps for k, v in res64["timing"].items() if v[1] > 0}}
         out["f32split_over_f64"] = value / v64
     if not args.no_ensemble:

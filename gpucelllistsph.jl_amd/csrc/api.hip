@@ -664,20 +664,26 @@ void LaunchSet::build(const int64_t* n, const int64_t* ncells, unsigned cnt, siz
   release();
   count = cnt;
   scene_bytes = bytes;
-  std::vector<int2> pm, cm;
+  std::vector<int2> pm, cm, pm512;
   for (unsigned i = 0; i < cnt; ++i) {
     const int64_t nb = (n[i] + kBlock - 1) / kBlock;
     for (int64_t b = 0; b < nb; ++b) pm.push_back(make_int2(int(i), int(b)));
+    const int64_t nb512 = (n[i] + 511) / 512;  // the pass kernels' 512-thread blocks
+    for (int64_t b = 0; b < nb512; ++b) pm512.push_back(make_int2(int(i), int(b)));
     const int64_t nt = (ncells[i] + 1 + 1023) / 1024;  // cnt[0..ncells] in 1024-cell tiles
     for (int64_t t = 0; t < nt; ++t) cm.push_back(make_int2(int(i), int(t)));
   }
   if (pm.empty()) pm.push_back(make_int2(0, 0));  // a no-op block (a >= n everywhere)
+  if (pm512.empty()) pm512.push_back(make_int2(0, 0));
   pblocks = unsigned(pm.size());
+  pblocks512 = unsigned(pm512.size());
   ctiles = unsigned(cm.size());
   GCL_HIP(hipMalloc(&scenes, bytes * cnt));
   GCL_HIP(hipMalloc(&pmap, sizeof(int2) * pm.size()));
+  GCL_HIP(hipMalloc(&pmap512, sizeof(int2) * pm512.size()));
   GCL_HIP(hipMalloc(&cmap, sizeof(int2) * cm.size()));
   GCL_HIP(hipMemcpy(pmap, pm.data(), sizeof(int2) * pm.size(), hipMemcpyHostToDevice));
+  GCL_HIP(hipMemcpy(pmap512, pm512.data(), sizeof(int2) * pm512.size(), hipMemcpyHostToDevice));
   GCL_HIP(hipMemcpy(cmap, cm.data(), sizeof(int2) * cm.size(), hipMemcpyHostToDevice));
 }
 
@@ -687,10 +693,11 @@ void LaunchSet::release() {
   graph_key.clear();
   if (scenes) (void)hipFree(scenes);
   if (pmap) (void)hipFree(pmap);
+  if (pmap512) (void)hipFree(pmap512);
   if (cmap) (void)hipFree(cmap);
   scenes = nullptr;
-  pmap = cmap = nullptr;
-  pblocks = ctiles = count = 0;
+  pmap = cmap = pmap512 = nullptr;
+  pblocks = ctiles = count = pblocks512 = 0;
 }
 
 namespace {
@@ -786,32 +793,38 @@ void run_steps(sph_s* const* P, unsigned count, LaunchSet& L, hipStream_t st, in
   };
   int64_t done = 0;
   const char* no_tune = std::getenv("GCLSPH_AUTOTUNE");
-  if (!forced && L.pass_variant[0] < 0 && !timer && nsteps > kPassVariants && !(no_tune && no_tune[0] == '0')) {
-    // Autotune: the first kPassVariants steps of the call run one variant each
-    // (live steps: every variant computes the same values), timing both
-    // passes; the fastest variant of each pass is kept for this launch set.
-    std::vector<hipEvent_t> ev(3 * kPassVariants, nullptr);
+  if (!forced && L.pass_variant[0] < 0 && !timer && nsteps > kTuneSteps && !(no_tune && no_tune[0] == '0')) {
+    // Autotune on the first kTuneSteps steps of the call (live steps: every
+    // variant computes the same values), timing both passes with events: the
+    // kPassCodegen code-generation variants at 256 threads per block first,
+    // then the fastest of them per pass at 128 and 512 threads.  The fastest
+    // variant of each pass is kept for this launch set.
+    std::vector<hipEvent_t> ev(3 * kTuneSteps, nullptr);
     for (auto& e : ev) GCL_HIP(hipEventCreate(&e));
-    for (int v = 0; v < kPassVariants; ++v) launch_step(cspm_at(v + 1), v, v, &ev[3 * size_t(v)]);
-    GCL_HIP(hipStreamSynchronize(st));
     float best[2] = {0, 0};
-    for (int v = 0; v < kPassVariants; ++v) {
+    auto time_step = [&](int k, int v1, int v2) {
+      launch_step(cspm_at(k + 1), v1, v2, &ev[3 * size_t(k)]);
+      GCL_HIP(hipStreamSynchronize(st));
       float t1 = 0, t2 = 0;
-      GCL_HIP(hipEventElapsedTime(&t1, ev[3 * size_t(v)], ev[3 * size_t(v) + 1]));
-      GCL_HIP(hipEventElapsedTime(&t2, ev[3 * size_t(v) + 1], ev[3 * size_t(v) + 2]));
-      if (v == 0 || t1 < best[0]) {
+      GCL_HIP(hipEventElapsedTime(&t1, ev[3 * size_t(k)], ev[3 * size_t(k) + 1]));
+      GCL_HIP(hipEventElapsedTime(&t2, ev[3 * size_t(k) + 1], ev[3 * size_t(k) + 2]));
+      if (k == 0 || t1 < best[0]) {
         best[0] = t1;
-        pv[0] = v;
+        pv[0] = v1;
       }
-      if (v == 0 || t2 < best[1]) {
+      if (k == 0 || t2 < best[1]) {
         best[1] = t2;
-        pv[1] = v;
+        pv[1] = v2;
       }
-    }
+    };
+    for (int v = 0; v < kPassCodegen; ++v) time_step(v, v, v);
+    const int cg1 = pv[0], cg2 = pv[1];
+    for (int b = 1; b < kPassBlockSizes; ++b)
+      time_step(kPassCodegen + b - 1, cg1 + kPassCodegen * b, cg2 + kPassCodegen * b);
     for (auto& e : ev) (void)hipEventDestroy(e);
     L.pass_variant[0] = pv[0];
     L.pass_variant[1] = pv[1];
-    done = kPassVariants;
+    done = kTuneSteps;
   }
   const char* no_graph = std::getenv("GCLSPH_NO_GRAPH");
   const bool graph = !timer && nsteps - done >= 2 && !(no_graph && no_graph[0] == '1');
@@ -1505,6 +1518,17 @@ int sph_get_timing(sph_t h, double* ms, int64_t* launches, int cap, int* nstages
   });
 }
 
+int sph_get_pass_variants(sph_t h, int32_t* out4) {
+  return guard([&] {
+    GCL_CHECK(h != nullptr && out4 != nullptr, kErrArgument, "NULL argument");
+    for (int k = 0; k < 2; ++k) {
+      const int v = h->own.pass_variant[k];
+      out4[k] = v;
+      out4[2 + k] = v >= 0 ? pass_block_size(v) : -1;
+    }
+  });
+}
+
 int sph_batch_create(const sph_t* problems, int32_t count, sph_batch_t* out) {
   return guard([&] {
     GCL_CHECK(problems != nullptr && out != nullptr, kErrArgument, "NULL argument");
@@ -1566,3 +1590,14 @@ int sph_batch_stepsolve(sph_batch_t b, int64_t n, int timestepping, const double
 }
 
 }  // extern "C"
+
+int sph_batch_get_pass_variants(sph_batch_t b, int32_t* out4) {
+  return guard([&] {
+    GCL_CHECK(b != nullptr && out4 != nullptr, kErrArgument, "NULL argument");
+    for (int k = 0; k < 2; ++k) {
+      const int v = b->L.pass_variant[k];
+      out4[k] = v;
+      out4[2 + k] = v >= 0 ? pass_block_size(v) : -1;
+    }
+  });
+}

@@ -205,7 +205,9 @@ struct SceneDev {
 
 // Launch geometry of a set of problems stepped together (a batch; a single
 // problem is a batch of one).  Particle kernels run one block per 256 particles
-// of one problem; pmap maps a logical block to (problem, block of the problem).
+// of one problem; pmap maps a logical block to (problem, block of the problem);
+// pmap512 does the same for the 512-thread blocks of the pass kernels (their
+// 128-thread blocks halve the pmap entries).
 // Cell kernels run one block per 1024-cell tile; cmap maps a block to (problem,
 // tile).  Per-problem kernels run one block per problem.
 struct LaunchSet {
@@ -213,7 +215,8 @@ struct LaunchSet {
   const void* one = nullptr;  // host SceneDev<R> passed by value when set (count == 1)
   int2* pmap = nullptr;
   int2* cmap = nullptr;
-  unsigned pblocks = 0, ctiles = 0, count = 0;
+  int2* pmap512 = nullptr;
+  unsigned pblocks = 0, ctiles = 0, count = 0, pblocks512 = 0;
   size_t scene_bytes = 0;
   // pass variants chosen by timing (-1: not yet)
   int pass_variant[2] = {-1, -1};
@@ -229,8 +232,17 @@ struct LaunchSet {
 // binning, scan, scatter, rank, reorder of the state, commit, membership store.
 template <int D, typename R>
 void launch_rebuild(const LaunchSet& L, hipStream_t s);
-// Pair passes in one of kPassVariants code-generation variants (same results).
-constexpr int kPassVariants = 4;
+// Pair passes in one of kPassVariants variants that differ only in code
+// generation and launch shape (same results): variant = cg + 4·bs, with cg in
+// [0, 4) the prefetch / occupancy choice and bs in [0, 3) the block size
+// (256, 128, 512 threads).
+constexpr int kPassCodegen = 4;
+constexpr int kPassBlockSizes = 3;
+constexpr int kPassVariants = kPassCodegen * kPassBlockSizes;
+// Steps the autotuner times: every code-generation variant at 256 threads,
+// then the best of them at each other block size.
+constexpr int kTuneSteps = kPassCodegen + kPassBlockSizes - 1;
+constexpr int pass_block_size(int variant) { return variant / kPassCodegen == 1 ? 128 : variant / kPassCodegen == 2 ? 512 : 256; }
 template <int D, typename R>
 void launch_pass1(const LaunchSet& L, hipStream_t s, int variant);
 template <int D, typename R>

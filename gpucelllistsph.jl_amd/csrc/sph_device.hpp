@@ -210,8 +210,11 @@ struct ManyScenes {
   const SceneDev<R>* __restrict__ S;
   const int2* __restrict__ pmap;
   const int2* __restrict__ cmap;
+  const int2* __restrict__ pmap512;  // (problem, 512-particle block of the problem)
 };
-template <typename R>
+// Blocks of BS threads (128, 256 or 512) over one problem's particles: m.y is
+// the block's index in units of BS particles, so particle a = m.y·BS + tid.
+template <int BS = kBlock, typename R>
 __device__ __forceinline__ const SceneDev<R>& particle_scene(const OneScene<R>& s, int2& m) {
   m = make_int2(0, int(xcd_block()));
   return s.d;
@@ -227,9 +230,18 @@ __device__ __forceinline__ const SceneDev<R>& batch_scene(const ManyScenes<R>& s
   CS* c = (CS*)(s.S);
   return *(const SceneDev<R>*)(c + i);
 }
-template <typename R>
+template <int BS = kBlock, typename R>
 __device__ __forceinline__ const SceneDev<R>& particle_scene(const ManyScenes<R>& s, int2& m) {
-  m = s.pmap[xcd_block()];
+  static_assert(BS == 128 || BS == 256 || BS == 512, "block sizes of the particle maps");
+  const unsigned L = xcd_block();
+  if constexpr (BS == 256) {
+    m = s.pmap[L];
+  } else if constexpr (BS == 128) {  // halves of the 256-particle blocks
+    const int2 e = s.pmap[L >> 1];
+    m = make_int2(e.x, e.y * 2 + int(L & 1u));
+  } else {
+    m = s.pmap512[L];
+  }
   return batch_scene(s, m.x);
 }
 template <typename R>
@@ -264,18 +276,20 @@ __device__ __forceinline__ StepFlags step_flags(const SceneDev<R>& sd) {
   return F;
 }
 
-// Binds the problem of this block: B, C, G, F and the particle index a.
-#define GCLSPH_SCENE(ss)                                       \
+// Binds the problem of this block of BS threads: B, C, G, F and the particle
+// index a.
+#define GCLSPH_SCENE_BS(ss, BS)                                \
   int2 m;                                                      \
-  const SceneDev<R>& sd = particle_scene<R>(ss, m);            \
+  const SceneDev<R>& sd = particle_scene<BS, R>(ss, m);        \
   const SphBufs<R>& B = sd.B;                                  \
   const SphConst& C = sd.C;                                    \
   const CellGeom& G = sd.G;                                    \
   const StepFlags F = step_flags(sd);                          \
-  const int64_t a = int64_t(m.y) * kBlock + threadIdx.x;       \
+  const int64_t a = int64_t(m.y) * (BS) + threadIdx.x;         \
   (void)C;                                                     \
   (void)G;                                                     \
   (void)F
+#define GCLSPH_SCENE(ss) GCLSPH_SCENE_BS(ss, kBlock)
 
 // Member bits of one window [lo, hi) (hi - lo <= 32) of a stencil row: bit j is
 // set when candidate lo + j is a partner of particle a, i.e. b != a and
@@ -540,15 +554,16 @@ constexpr int kRedSlots = 64;
 // on its bit pattern (non-negative doubles order like their bits, NaN above
 // +Inf); spreading blocks over many slots avoids one hot address.  Every
 // thread of the block must call it.
+template <int BS = kBlock>
 __device__ __forceinline__ void block_max(double v, bool active, double* slots, int blk) {
-  __shared__ double wm[kBlock / 64];
+  __shared__ double wm[BS / 64];
   double m = wave_max(active ? v : 0.0);
   if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
   __syncthreads();
   if (threadIdx.x == 0) {
     double r = wm[0];
 #pragma unroll
-    for (int w = 1; w < kBlock / 64; ++w) r = nanmax(r, wm[w]);
+    for (int w = 1; w < BS / 64; ++w) r = nanmax(r, wm[w]);
     atomic_max_nonneg(reinterpret_cast<unsigned long long*>(slots) + (blk % kRedSlots), r);
   }
   __syncthreads();
@@ -559,7 +574,7 @@ __device__ __forceinline__ void block_max(double v, bool active, double* slots, 
 template <typename R, typename Fn>
 inline void with_scenes(const LaunchSet& L, Fn&& f) {
   if (L.one) f(OneScene<R>{*static_cast<const SceneDev<R>*>(L.one)});
-  else f(ManyScenes<R>{static_cast<const SceneDev<R>*>(L.scenes), L.pmap, L.cmap});
+  else f(ManyScenes<R>{static_cast<const SceneDev<R>*>(L.scenes), L.pmap, L.cmap, L.pmap512});
 }
 
 }  // namespace gclsph

@@ -25,17 +25,18 @@
 
 namespace gclsph {
 
-// Pass kernels come in variants that differ only in code generation (so
-// their results are bit-identical): Pipe fetches the next partner's data
-// before the current pair body (two register sets), W asks the compiler for at
-// least W waves per SIMD (fewer registers).  The step driver picks one per
-// pass by timing them on live steps.
+// Pass kernels come in variants that differ only in code generation and
+// launch shape (so their results are bit-identical: one thread computes one
+// particle whatever the block): Pipe fetches the next partner's data before the
+// current pair body (two register sets), W asks the compiler for at least W
+// waves per SIMD (fewer registers), BS is the block size.  The step driver
+// picks one per pass by timing them on live steps.
 //
 // ---- pass 1 ---------------------------------------------------------------
-template <int D, typename R, typename SS, bool Pipe, int W>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_pass1(SS ss) {
+template <int D, typename R, typename SS, bool Pipe, int W, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_pass1(SS ss) {
 #pragma clang fp contract(off)
-  GCLSPH_SCENE(ss);
+  GCLSPH_SCENE_BS(ss, BS);
   const StepCtl* ctl = B.ctl;
   const int p = ctl->parity, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
@@ -293,10 +294,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
 }
 
 // ---- pass 2 + symplectic update ------------------------------------------
-template <int D, typename R, typename SS, bool Pipe, int W>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k_pass2(SS ss) {
+template <int D, typename R, typename SS, bool Pipe, int W, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_pass2(SS ss) {
 #pragma clang fp contract(off)
-  GCLSPH_SCENE(ss);
+  GCLSPH_SCENE_BS(ss, BS);
   StepCtl* ctl = B.ctl;
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, rebuilt = ctl->rebuilt;
   const double dt = ctl->dt;
@@ -603,43 +604,68 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W))) void k
       B.PBUF[a] = Pha;
     }
   }
-  block_max(red_cdx, active, B.RED, m.y);
+  block_max<BS>(red_cdx, active, B.RED, m.y);
   if (F.ts) {
-    block_max(red_acc, active, B.RED + kRedSlots, m.y);
-    if (F.visc_in_pass2) block_max(red_visc, active, B.RED + 2 * kRedSlots, m.y);
+    block_max<BS>(red_acc, active, B.RED + kRedSlots, m.y);
+    if (F.visc_in_pass2) block_max<BS>(red_visc, active, B.RED + 2 * kRedSlots, m.y);
   }
 }
 
 // ------------------------------------------------------------ host launchers
 namespace {
-template <int D, typename R, bool Pipe, int W>
-void pass1_as(const LaunchSet& L, hipStream_t s) {
-  with_scenes<R>(L, [&](auto ss) { k_pass1<D, R, decltype(ss), Pipe, W><<<L.pblocks, kBlock, 0, s>>>(ss); });
+// Blocks of BS threads covering every problem of L: the 256-particle blocks of
+// the particle map, twice as many 128-thread blocks, or the 512-particle map.
+template <int BS>
+unsigned pass_grid(const LaunchSet& L) {
+  return BS == 256 ? L.pblocks : BS == 128 ? 2 * L.pblocks : L.pblocks512;
 }
-template <int D, typename R, bool Pipe, int W>
+template <int D, typename R, bool Pipe, int W, int BS>
+void pass1_as(const LaunchSet& L, hipStream_t s) {
+  with_scenes<R>(L, [&](auto ss) { k_pass1<D, R, decltype(ss), Pipe, W, BS><<<pass_grid<BS>(L), BS, 0, s>>>(ss); });
+}
+template <int D, typename R, bool Pipe, int W, int BS>
 void pass2_as(const LaunchSet& L, hipStream_t s) {
-  with_scenes<R>(L, [&](auto ss) { k_pass2<D, R, decltype(ss), Pipe, W><<<L.pblocks, kBlock, 0, s>>>(ss); });
+  with_scenes<R>(L, [&](auto ss) { k_pass2<D, R, decltype(ss), Pipe, W, BS><<<pass_grid<BS>(L), BS, 0, s>>>(ss); });
+}
+// code generation cg: 0 pipelined; 1 not pipelined; 2 pipelined, >= 3
+// waves/SIMD; 3 not pipelined, >= 5 waves/SIMD
+template <int D, typename R, int BS>
+void pass1_cg(const LaunchSet& L, hipStream_t s, int cg) {
+  switch (cg) {
+    case 1: pass1_as<D, R, false, 1, BS>(L, s); break;
+    case 2: pass1_as<D, R, true, 3, BS>(L, s); break;
+    case 3: pass1_as<D, R, false, 5, BS>(L, s); break;
+    default: pass1_as<D, R, true, 1, BS>(L, s); break;
+  }
+}
+template <int D, typename R, int BS>
+void pass2_cg(const LaunchSet& L, hipStream_t s, int cg) {
+  switch (cg) {
+    case 1: pass2_as<D, R, false, 1, BS>(L, s); break;
+    case 2: pass2_as<D, R, true, 3, BS>(L, s); break;
+    case 3: pass2_as<D, R, false, 5, BS>(L, s); break;
+    default: pass2_as<D, R, true, 1, BS>(L, s); break;
+  }
 }
 }  // namespace
 
-// variant: 0 pipelined; 1 not pipelined; 2 pipelined, >= 3 waves/SIMD;
-// 3 not pipelined, >= 5 waves/SIMD
+// variant = cg + kPassCodegen·bs (bs 0: 256, 1: 128, 2: 512 threads per block)
 template <int D, typename R>
 void launch_pass1(const LaunchSet& L, hipStream_t s, int variant) {
-  switch (variant) {
-    case 1: pass1_as<D, R, false, 1>(L, s); break;
-    case 2: pass1_as<D, R, true, 3>(L, s); break;
-    case 3: pass1_as<D, R, false, 5>(L, s); break;
-    default: pass1_as<D, R, true, 1>(L, s); break;
+  const int cg = variant % kPassCodegen;
+  switch (pass_block_size(variant)) {
+    case 128: pass1_cg<D, R, 128>(L, s, cg); break;
+    case 512: pass1_cg<D, R, 512>(L, s, cg); break;
+    default: pass1_cg<D, R, 256>(L, s, cg); break;
   }
 }
 template <int D, typename R>
 void launch_pass2(const LaunchSet& L, hipStream_t s, int variant) {
-  switch (variant) {
-    case 1: pass2_as<D, R, false, 1>(L, s); break;
-    case 2: pass2_as<D, R, true, 3>(L, s); break;
-    case 3: pass2_as<D, R, false, 5>(L, s); break;
-    default: pass2_as<D, R, true, 1>(L, s); break;
+  const int cg = variant % kPassCodegen;
+  switch (pass_block_size(variant)) {
+    case 128: pass2_cg<D, R, 128>(L, s, cg); break;
+    case 512: pass2_cg<D, R, 512>(L, s, cg); break;
+    default: pass2_cg<D, R, 256>(L, s, cg); break;
   }
 }
 

@@ -84,6 +84,8 @@ def lib():
         "sph_batch_create": [C.POINTER(P), C.c_int32, C.POINTER(P)],
         "sph_batch_destroy": [P],
         "sph_batch_stepsolve": [P, C.c_int64, C.c_int, dp, dp, i64p, dp],
+        "sph_get_pass_variants": [P, C.POINTER(C.c_int32)],
+        "sph_batch_get_pass_variants": [P, C.POINTER(C.c_int32)],
         "gcl_neighborlist_noalloc": [C.c_int, C.c_int64, C.POINTER(C.c_void_p), dp, C.c_double, dp, dp, C.c_void_p,
                                      C.c_int64, i64p, C.c_void_p],
     }
@@ -105,4 +107,5 @@ EXPORTED = ("gclsph_last_error", "gclsph_version", "gcl_create", "gcl_destroy", 
             "gcl_set_points", "gcl_get_points", "gcl_pairsn", "gcl_capacity", "gcl_get_pairs", "gcl_info",
             "gcl_get_cellpnum", "gcl_get_pcell", "gcl_maxpairs", "sph_create", "sph_destroy", "sph_set_param",
             "sph_get_param", "sph_stepsolve", "sph_get_field", "sph_set_timing", "sph_get_timing", "sph_batch_create",
-            "sph_batch_destroy", "sph_batch_stepsolve", "gcl_neighborlist_noalloc")
+            "sph_batch_destroy", "sph_batch_stepsolve", "gcl_neighborlist_noalloc", "sph_get_pass_variants",
+            "sph_batch_get_pass_variants")

@@ -327,6 +327,15 @@ class SPHProblem:
         names = ("rebuild", "pass1", "pass2", "corrections", "finalize")
         return {names[k]: (ms[k], int(cnt[k])) for k in range(ns.value)}
 
+    def pass_variants(self) -> dict:
+        """The pair-pass variants the autotuner chose for this problem's own
+        calls (variant = codegen + 4·blocksize index) and their block sizes;
+        -1 until tuned."""
+        out = (C.c_int32 * 4)()
+        check(lib().sph_get_pass_variants(self._h, out))
+        return {"pass1": int(out[0]), "pass2": int(out[1]), "pass1_threads": int(out[2]),
+                "pass2_threads": int(out[3])}
+
     def __repr__(self):
         s = self.s
         dpc = f"l₀ = {self.dpc_l0} , λ = {self.dpc_lambda}" if self.dpc_l0 > 0 and self.dpc_lambda > 0 else "not used"
@@ -368,6 +377,14 @@ class SPHBatch:
 
     def __len__(self):
         return len(self.problems)
+
+    def pass_variants(self) -> dict:
+        """The pair-pass variants the autotuner chose for this batch (see
+        SPHProblem.pass_variants)."""
+        out = (C.c_int32 * 4)()
+        check(lib().sph_batch_get_pass_variants(self._h, out))
+        return {"pass1": int(out[0]), "pass2": int(out[1]), "pass1_threads": int(out[2]),
+                "pass2_threads": int(out[3])}
 
     def __del__(self):
         h = self.__dict__.get("_h")

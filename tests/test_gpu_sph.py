@@ -292,25 +292,32 @@ def test_full_size_f32split_vs_f64_1000_steps(G, S):
         cl = G.GPUCellList(pts, sc.cellsize, sc.dist)
         prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, precision=prec)
         G.stepsolve_(prob, 1000, timestepping=True)
-        out = (np.array(G.get_points(prob)), G.get_density(prob), prob.etime)
+        out = (np.array(G.get_points(prob)), G.get_density(prob), prob.etime, np.array(G.get_velocity(prob)))
         del prob
         return out
 
-    x64, r64, t64 = run("f64")
-    xp, _, _ = run("f64", 4e-16)
-    x32, r32, t32 = run("f32split")
+    x64, r64, t64, v64 = run("f64")
+    xp, _, _, vp = run("f64", 4e-16)
+    x32, r32, t32, v32 = run("f32split")
     dx = sc.params.dx
     chaos = np.abs(xp - x64).max(axis=0)
     floor = chaos.max()
     err = np.abs(x32 - x64).max(axis=0)
+    vchaos = np.abs(vp - v64).max(axis=0)
+    verr = np.abs(v32 - v64).max(axis=0)
     print(f"density max rel {np.abs(r32 / r64 - 1).max():.3e}; position/dx max {err.max() / dx:.3e} "
           f"p99.99 {np.quantile(err, 0.9999) / dx:.3e}; FP64 chaos floor/dx {floor / dx:.3e} "
-          f"p99.99 {np.quantile(chaos, 0.9999) / dx:.3e}")
+          f"p99.99 {np.quantile(chaos, 0.9999) / dx:.3e}; velocity max {verr.max():.3e} "
+          f"p99.99 {np.quantile(verr, 0.9999):.3e} (FP64 floor {vchaos.max():.3e} / "
+          f"{np.quantile(vchaos, 0.9999):.3e}) m/s")
     assert np.abs(r32 / r64 - 1).max() < 1e-5
     assert np.quantile(err, 0.9999) < 1e-4 * dx
-    # FP32-split deviates less than FP64 does from its own 1-ulp perturbation
+    # FP32-split deviates less than FP64 does from its own 1-ulp perturbation,
+    # in positions and in velocities
     assert np.quantile(err, 0.9999) <= np.quantile(chaos, 0.9999)
     assert err.max() <= floor
+    assert np.quantile(verr, 0.9999) <= np.quantile(vchaos, 0.9999)
+    assert verr.max() <= vchaos.max()
     assert t32 == pytest.approx(t64, rel=1e-6)
 
 
@@ -375,9 +382,10 @@ def test_config2_3d_f32split_vs_f64_500_steps(G, S):
 
 @pytest.mark.parametrize("prec", ["f64", "f32split"])
 def test_pass_variants_bit_identical(G, monkeypatch, prec):
-    """The pair passes exist in code-generation variants (partner prefetch,
-    occupancy target) that the step driver times on live steps and picks from;
-    every variant, forced or autotuned, must give bit-identical steps."""
+    """The pair passes exist in variants (partner prefetch, occupancy target,
+    and 128, 256 or 512 threads per block: variant = codegen + 4·blocksize)
+    that the step driver times on live steps and picks from; every variant,
+    forced or autotuned, must give bit-identical steps."""
     sc, steps, ts = golden_cases()["dam2d_allphys"]
 
     def run(variant):
@@ -388,10 +396,15 @@ def test_pass_variants_bit_identical(G, monkeypatch, prec):
         cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
         prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, v=sc.v, precision=prec)
         out = [G.stepsolve_(prob, 9, timestepping=ts), G.stepsolve_(prob, 3, timestepping=ts)]
+        if variant is None:  # the tuner ran on the first call and recorded its choice
+            pv = prob.pass_variants()
+            assert 0 <= pv["pass1"] < 12 and 0 <= pv["pass2"] < 12
+            assert pv["pass1_threads"] in (128, 256, 512) and pv["pass2_threads"] in (128, 256, 512)
         return out + [np.asarray(gpu_fields(G, prob)[f]) for f in FIELDS]
 
     ref = run("0,0")
-    for v in ("1,1", "2,2", "3,3", "1,3", None):
+    for v in ("1,1", "2,2", "3,3", "1,3", "4,4", "5,5", "6,6", "7,7", "8,8", "9,9", "10,10", "11,11", "4,9", "8,1",
+              None):
         got = run(v)
         for a, b in zip(got, ref):
             assert np.array_equal(np.asarray(a), np.asarray(b)), v
