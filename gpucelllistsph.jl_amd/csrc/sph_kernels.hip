@@ -262,17 +262,13 @@ __global__ void k_post_rebuild(SS ss) {
   }
 }
 
-// New sorted position s takes the state of old position perm[s].
-template <int D, typename R, typename SS>
-__global__ void k_reorder(SS ss) {
-  GCLSPH_SCENE(ss);
-  const StepCtl* ctl = B.ctl;
-  if (ctl->rebuild == 0) return;
-  const int64_t s = a;
-  if (s >= B.n) return;
+// New sorted position s takes the state of old position perm[s]; returns the
+// particle's position (xs) for a membership test that follows.
+template <int D, typename R>
+__device__ __forceinline__ void reorder_one(const SphBufs<R>& B, const SphConst& C, const StepFlags& F,
+                                            const StepCtl* ctl, int64_t s, double* xs) {
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, pq = pp ^ 1;
   const int32_t src = B.perm[s];
-  double xs[3];
   R vs[3];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
@@ -310,6 +306,126 @@ __global__ void k_reorder(SS ss) {
   if (!F.synced_prev) B.RH[s] = B.RH2[src];
   B.PT[pq][s] = pt;
   B.ID[pq][s] = B.ID[pp][src];
+}
+
+template <int D, typename R, typename SS>
+__global__ void k_reorder(SS ss) {
+  GCLSPH_SCENE(ss);
+  const StepCtl* ctl = B.ctl;
+  if (ctl->rebuild == 0) return;
+  if (a >= B.n) return;
+  double xs[3];
+  reorder_one<D, R>(B, C, F, ctl, a, xs);
+}
+
+// window_bits for candidates read through the new order: candidate b (a
+// sorted index) sits at old position perm[b] of x0/x1.  Reads past n (the
+// window reads run up to 3 past a row's end) are clamped to n - 1; their bits
+// are dropped.
+__device__ __forceinline__ uint32_t window_bits_perm(const double* x0, const double* x1, const int32_t* perm,
+                                                     int64_t n, double xa0, double xa1, int64_t a, int32_t lo,
+                                                     int32_t hi, double dist2) {
+#pragma clang fp contract(off)
+  const int32_t len = hi - lo;
+  const int32_t last = int32_t(n - 1);
+  uint32_t acc = 0;
+  for (int32_t j = 0; j < len; j += 4) {
+    double r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int32_t b = lo + j + u < last ? lo + j + u : last;
+      const int32_t src = perm[b];
+      const double d0 = xa0 - x0[src];
+      const double d1 = xa1 - x1[src];
+      r[u] = d0 * d0 + d1 * d1;
+    }
+    acc = push_lt4(acc, r[0], r[1], r[2], r[3], dist2);
+  }
+  return window_mask(acc, len, a, lo);
+}
+
+// Membership store (k_nmask's 2D path) of particle a, reading the rebuild
+// positions through the new order from the positions before the reorder
+// (x_old[perm[b]] is the reordered x[b]), so it needs no other block's reorder
+// writes.  xa: a's rebuild position.  Wave-level only (no block barrier).
+template <typename R>
+__device__ __forceinline__ void nmask2_perm(const SphBufs<R>& B, const SphConst& C, const CellGeom& G,
+                                            const StepCtl* ctl, int64_t a, int64_t a0, const double* xa,
+                                            double2 (*lds)[kNmaskWave + 4]) {
+  const bool active = a < B.n;
+  const double dist2 = C.dist2;
+  const int p = ctl->parity;  // positions before the reorder
+  const double* XO[2] = {B.X[p][0], B.X[p][1]};
+  const int32_t* perm = B.perm;
+  const int cap = B.necap;
+  uint64_t* out = B.NE + a;
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int lane = int(threadIdx.x & 63);
+  const int64_t aw = a0 + 64 * w;
+  if (aw >= B.n) return;  // whole wave
+  RowUnion u;
+  row_union(u, B.start, B.skey, G, B.n, aw, 0, 64);
+  const int32_t total = u.uo[3];
+  int cnt = 0;
+  if (total <= kNmaskWave) {  // uniform over the wave
+    double2* wl = lds[w];
+    const int32_t d0 = u.ub[0] - u.uo[0], d1 = u.ub[1] - u.uo[1], d2 = u.ub[2] - u.uo[2];
+    for (int32_t i = lane; i < total; i += 64) {
+      const int32_t b = i + (i >= u.uo[2] ? d2 : (i >= u.uo[1] ? d1 : d0));
+      const int32_t src = perm[b];
+      wl[i] = make_double2(XO[0][src], XO[1][src]);
+    }
+    // the wave's LDS operations complete in issue order; the fence keeps the
+    // compiler from moving the reads below ahead of these writes
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (active) {
+      const Rows<2> rows = stencil_rows<2>(G, B.start, B.skey[a]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double2* lrow = wl + (u.uo[k] - u.ub[k]);  // candidate b at lrow[b]
+        for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
+          const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
+          const uint32_t mk = window_bits_lds(lrow + w0, xa[0], xa[1], a, w0, hi, dist2);
+          if (mk != 0) {
+            if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
+            ++cnt;
+          }
+        }
+      }
+    }
+  } else if (active) {
+    const Rows<2> rows = stencil_rows<2>(G, B.start, B.skey[a]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
+        const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
+        const uint32_t mk = window_bits_perm(XO[0], XO[1], perm, B.n, xa[0], xa[1], a, w0, hi, dist2);
+        if (mk != 0) {
+          if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
+          ++cnt;
+        }
+      }
+    }
+  }
+  if (active) B.NC[a] = cnt <= cap ? cnt : -1;
+}
+
+// 2D rebuild tail in one kernel: the reorder of the state and the membership
+// store of the same particles, which are independent of each other (one
+// launch instead of two; both parts are bound by vector-memory traffic, so
+// they add up rather than overlap).  Same tests, same bits as k_nmask.
+template <typename R, typename SS>
+__global__ __launch_bounds__(256) void k_reorder_nmask2(SS ss) {
+  GCLSPH_SCENE(ss);
+  const StepCtl* ctl = B.ctl;
+  if (ctl->rebuild == 0) return;
+  const int64_t a0 = int64_t(m.y) * kBlock;
+  if (a0 >= B.n) return;  // whole block
+  __shared__ double2 lds[kBlock / 64][kNmaskWave + 4];  // +4: window reads run up to 3 past a row's end
+  double xa[3] = {0.0, 0.0, 0.0};
+  if (a < B.n) reorder_one<2, R>(B, C, F, ctl, a, xa);
+  nmask2_perm<R>(B, C, G, ctl, a, a0, xa, lds);
 }
 
 // ---- corrections after the update (src/sphproblem.jl:303-322) --------------
@@ -930,9 +1046,14 @@ void launch_rebuild(const LaunchSet& L, hipStream_t s) {
     kb_tile_scan<R, SS><<<L.ctiles, kScanThreads, 0, s>>>(ss);
     kb_scatter<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
     kb_rank<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
-    k_reorder<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
-    k_post_rebuild<R, SS><<<L.count, 64, 0, s>>>(ss);
-    k_nmask<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
+    if constexpr (D == 2) {
+      k_reorder_nmask2<R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
+      k_post_rebuild<R, SS><<<L.count, 64, 0, s>>>(ss);
+    } else {
+      k_reorder<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
+      k_post_rebuild<R, SS><<<L.count, 64, 0, s>>>(ss);
+      k_nmask<D, R, SS><<<L.pblocks, kBlock, 0, s>>>(ss);
+    }
   });
 }
 template <int D, typename R>
