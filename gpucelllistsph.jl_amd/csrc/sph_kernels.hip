@@ -106,16 +106,15 @@ constexpr int kNmaskPlane = 1280;
 // entry count, or -1 when the particle needs more than necap entries (its
 // partners are then found by scanning the stencil).
 //
-// 2D: each wave stages the positions of its own row union (row_union over its
-// 64 particles) in its LDS slice (coalesced) and every lane tests its
-// candidates there, instead of each lane fetching each of its ~100 candidates
-// through the vector L1; waves do not wait for each other.  3D: the block
-// stages plane by plane (three stencil rows at a time, ~1000 candidates per
-// particle in all).  A wave (2D) or block (3D) whose union exceeds the LDS
-// budget reads the candidates from global memory.  The tests are the same
-// expression in every path, so the bits are the same.
+// k_nmask is the 3D store (the 2D one runs inside k_reorder_nmask2 below):
+// the block stages the candidates' positions plane by plane (three stencil
+// rows at a time, ~1000 candidates per particle in all) and every lane tests
+// its candidates there; a block whose union exceeds the LDS budget reads them
+// from global memory.  The tests are the same expression in every path, so
+// the bits are the same.
 template <int D, typename R, typename SS>
 __global__ __launch_bounds__(256) void k_nmask(SS ss) {
+  static_assert(D == 3, "the 2D membership store is part of k_reorder_nmask2");
   GCLSPH_SCENE(ss);
   const StepCtl* ctl = B.ctl;
   if (!ctl->rebuilt) return;
@@ -127,116 +126,56 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
   const double* XM[3] = {B.X[p][0], B.X[p][1], B.X[p][D == 3 ? 2 : 1]};
   const int cap = B.necap;
   uint64_t* out = B.NE + a;
-  if constexpr (D == 2) {
-    __shared__ double2 lds[kBlock / 64][kNmaskWave + 4];  // +4: window reads run up to 3 past a row's end
-    const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
-    const int lane = int(threadIdx.x & 63);
-    const int64_t aw = a0 + 64 * w;
-    if (aw >= B.n) return;  // whole wave; no block barrier follows
-    RowUnion u;
-    row_union(u, B.start, B.skey, G, B.n, aw, 0, 64);
-    const int32_t total = u.uo[3];
-    if (total <= kNmaskWave) {  // uniform over the wave
-      double2* wl = lds[w];
-      // slot i holds candidate i + d_k of row k (selects, not an indexed
-      // array: the union lives in registers)
-      const int32_t d0 = u.ub[0] - u.uo[0], d1 = u.ub[1] - u.uo[1], d2 = u.ub[2] - u.uo[2];
-      for (int32_t i = lane; i < total; i += 64) {
-        const int32_t b = i + (i >= u.uo[2] ? d2 : (i >= u.uo[1] ? d1 : d0));
-        wl[i] = make_double2(XM[0][b], XM[1][b]);
-      }
-      // the wave's LDS operations complete in issue order, so its lanes read
-      // what the others wrote above; the fence keeps the compiler from moving
-      // the reads ahead of the writes
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      if (!active) return;
-      const double xa0 = XM[0][a], xa1 = XM[1][a];
-      const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
-      int cnt = 0;
+  __shared__ double lx[kNmaskPlane + 4], ly[kNmaskPlane + 4], lz[kNmaskPlane + 4];  // +4: window reads run up to 3 past a row's end
+  __shared__ RowUnion u;
+  double xa0 = 0.0, xa1 = 0.0, xa2 = 0.0;
+  Rows<D> rows{};
+  if (active) {
+    xa0 = XM[0][a];
+    xa1 = XM[1][a];
+    xa2 = XM[2][a];
+    rows = stencil_rows<D>(G, B.start, B.skey[a]);
+  }
+  int cnt = 0;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const double2* lrow = wl + (u.uo[k] - u.ub[k]);  // candidate b at lrow[b]
-        for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
-          const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
-          const uint32_t mk = window_bits_lds(lrow + w0, xa0, xa1, a, w0, hi, dist2);
+  for (int pl = 0; pl < 3; ++pl) {  // rows 3·pl .. 3·pl+2 lie in the plane dz = pl - 1
+    if (threadIdx.x == 0) row_union(u, B.start, B.skey, G, B.n, a0, pl - 1);
+    __syncthreads();
+    const int32_t total = u.uo[3];
+    const bool staged = total <= kNmaskPlane;  // uniform over the block
+    if (staged) {
+      for (int32_t i = threadIdx.x; i < total; i += kBlock) {
+        const int k = (i >= u.uo[1]) + (i >= u.uo[2]);
+        const int32_t b = u.ub[k] + (i - u.uo[k]);
+        lx[i] = XM[0][b];
+        ly[i] = XM[1][b];
+        lz[i] = XM[2][b];
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int kk = 0; kk < 3; ++kk) {
+        const int k = 3 * pl + kk;
+        const int32_t off = u.uo[kk] - u.ub[kk];  // candidate b at l?[b + off]
+        const int32_t rb = rows.b[k], re = rows.e[k];
+        for (int32_t w0 = rb; w0 < re; w0 += 32) {
+          const int32_t hi = re < w0 + 32 ? re : w0 + 32;
+          const uint32_t mk =
+              staged ? window_bits_lds3(lx + (w0 + off), ly + (w0 + off), lz + (w0 + off), xa0, xa1, xa2, a, w0, hi,
+                                        dist2)
+                     : window_bits<D>(XM[0], XM[1], XM[2], xa0, xa1, xa2, a, w0, hi, dist2);
           if (mk != 0) {
             if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
             ++cnt;
           }
         }
       }
-      B.NC[a] = cnt <= cap ? cnt : -1;
-      return;
     }
-  } else {
-    __shared__ double lx[kNmaskPlane + 4], ly[kNmaskPlane + 4], lz[kNmaskPlane + 4];  // +4 as in 2D
-    __shared__ RowUnion u;
-    double xa0 = 0.0, xa1 = 0.0, xa2 = 0.0;
-    Rows<D> rows{};
-    if (active) {
-      xa0 = XM[0][a];
-      xa1 = XM[1][a];
-      xa2 = XM[2][a];
-      rows = stencil_rows<D>(G, B.start, B.skey[a]);
-    }
-    int cnt = 0;
-#pragma unroll
-    for (int pl = 0; pl < 3; ++pl) {  // rows 3·pl .. 3·pl+2 lie in the plane dz = pl - 1
-      if (threadIdx.x == 0) row_union(u, B.start, B.skey, G, B.n, a0, pl - 1);
-      __syncthreads();
-      const int32_t total = u.uo[3];
-      const bool staged = total <= kNmaskPlane;  // uniform over the block
-      if (staged) {
-        for (int32_t i = threadIdx.x; i < total; i += kBlock) {
-          const int k = (i >= u.uo[1]) + (i >= u.uo[2]);
-          const int32_t b = u.ub[k] + (i - u.uo[k]);
-          lx[i] = XM[0][b];
-          ly[i] = XM[1][b];
-          lz[i] = XM[2][b];
-        }
-      }
-      __syncthreads();
-      if (active) {
-#pragma unroll
-        for (int kk = 0; kk < 3; ++kk) {
-          const int k = 3 * pl + kk;
-          const int32_t off = u.uo[kk] - u.ub[kk];  // candidate b at l?[b + off]
-          const int32_t rb = rows.b[k], re = rows.e[k];
-          for (int32_t w0 = rb; w0 < re; w0 += 32) {
-            const int32_t hi = re < w0 + 32 ? re : w0 + 32;
-            const uint32_t mk =
-                staged ? window_bits_lds3(lx + (w0 + off), ly + (w0 + off), lz + (w0 + off), xa0, xa1, xa2, a, w0, hi,
-                                          dist2)
-                       : window_bits<D>(XM[0], XM[1], XM[2], xa0, xa1, xa2, a, w0, hi, dist2);
-            if (mk != 0) {
-              if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
-              ++cnt;
-            }
-          }
-        }
-      }
-      __syncthreads();  // the next plane overwrites the staged positions
-    }
-    if (active) B.NC[a] = cnt <= cap ? cnt : -1;
-    return;
+    __syncthreads();  // the next plane overwrites the staged positions
   }
-  if (!active) return;
-  const double xa0 = XM[0][a], xa1 = XM[1][a], xa2 = D == 3 ? XM[2][a] : 0.0;
-  const Rows<D> rows = stencil_rows<D>(G, B.start, B.skey[a]);
-  int cnt = 0;
-#pragma unroll
-  for (int k = 0; k < Rows<D>::N; ++k) {
-    for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
-      const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
-      const uint32_t mk = window_bits<D>(XM[0], XM[1], XM[2], xa0, xa1, xa2, a, w0, hi, dist2);
-      if (mk != 0) {
-        if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
-        ++cnt;
-      }
-    }
-  }
-  B.NC[a] = cnt <= cap ? cnt : -1;
+  if (active) B.NC[a] = cnt <= cap ? cnt : -1;
+  return;
 }
 
 // ------------------------------------------------------------------ kernels
