@@ -159,6 +159,107 @@ __global__ void k_pair_count(XPtrs xsp, const int32_t* __restrict__ start, const
   cnt[s] = c;
 }
 
+// 2D count pass that also stores the windows holding neighbours as
+// (base << 32 | bits) entries, entry e of item s at ent[e*n + s], nent[s] of
+// them (-1 when more than entcap), so the emit pass reads them instead of
+// testing the candidates again.  Each wave stages the positions of rows
+// 1 and 2 of its stencil-row union (row_union over its 64 items; the half
+// stencil never reaches row 0, whose items all sort before the wave's) in its
+// LDS slice, coalesced, and every lane tests its candidates there; a wave
+// whose union exceeds kCountWave reads them from global memory.  The test is
+// pair_window's expression either way, so the bits are the same.
+constexpr int kCountWave = 512;
+
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_pair_count2(XPtrs xsp, const int32_t* __restrict__ start,
+                                                     const int32_t* __restrict__ skey, int64_t n, CellGeom g,
+                                                     T dist2max, int32_t* __restrict__ cnt,
+                                                     uint64_t* __restrict__ ent, int32_t* __restrict__ nent,
+                                                     int entcap) {
+#pragma clang fp contract(off)
+  using T2 = typename Vec2<T>::type;
+  __shared__ T2 lds[kBlock / 64][kCountWave + 4];  // +4: window reads run up to 3 past a row's end
+  const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
+  const int lane = int(threadIdx.x & 63);
+  const int64_t s0 = int64_t(blockIdx.x) * kBlock + 64 * w;
+  if (s0 >= n) return;  // whole wave; no block barrier follows
+  const int64_t s = s0 + lane;
+  const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
+                   static_cast<const T*>(xsp.p[1])};
+  RowUnion u;
+  row_union(u, start, skey, g, n, s0, 0, 64);
+  const int32_t total = u.uo[3] - u.uo[1];  // rows 1 and 2
+  const bool staged = total <= kCountWave;   // uniform over the wave
+  T2* wl = lds[w];
+  if (staged) {
+    const int32_t d1 = u.ub[1] - u.uo[1], d2 = u.ub[2] - u.uo[2];
+    for (int32_t i = lane; i < total; i += 64) {
+      const int32_t j = i + u.uo[1];
+      const int32_t b = j + (j >= u.uo[2] ? d2 : d1);
+      T2 v;
+      v.x = x[0][b];
+      v.y = x[1][b];
+      wl[i] = v;
+    }
+    // the wave's LDS operations complete in issue order; the fence keeps the
+    // compiler from moving the reads below ahead of these writes
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (s >= n) return;
+  const T xa0 = x[0][s], xa1 = x[1][s];
+  const Rows<2> r = stencil_rows<2>(g, start, skey[s]);
+  int32_t c = 0;
+  int ne = 0;
+#pragma unroll
+  for (int k = 1; k < 3; ++k) {  // row 0 holds no candidate t > s
+    const T2* lrow = wl + (u.uo[k] - u.uo[1] - u.ub[k]);  // candidate t at lrow[t]
+    const int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
+    for (int32_t w0 = b; w0 < r.e[k]; w0 += 32) {
+      const int32_t hi = r.e[k] < w0 + 32 ? r.e[k] : w0 + 32;
+      uint32_t mk;
+      if (staged) {
+        const int32_t len = hi - w0;
+        mk = 0;
+        for (int32_t j = 0; j < len; j += 4) {
+          T2 cv[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) cv[q] = lrow[w0 + j + q];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const T d0 = xa0 - cv[q].x;
+            const T d1 = xa1 - cv[q].y;
+            const T r2 = d0 * d0 + d1 * d1;
+            mk |= uint32_t(r2 < dist2max) << (j + q);
+          }
+        }
+        if (len < 32) mk &= (1u << len) - 1u;
+      } else {
+        mk = pair_window<T, 2>(x, xa0, xa1, T(0), w0, hi, dist2max);
+      }
+      c += __builtin_popcount(mk);
+      if (mk != 0) {
+        if (ne < entcap)
+          ent[int64_t(ne) * n + s] = (uint64_t(uint32_t(w0)) << 32) | mk;
+        ++ne;
+      }
+    }
+  }
+  cnt[s] = c;
+  nent[s] = ne <= entcap ? ne : -1;
+}
+
 // Pairs staged in LDS per wave before the wave writes them out: a wave's
 // items are consecutive, so its pairs fill one contiguous range of the output
 // [off[s0], off[s0 + 64]); staged, that range goes out in whole-line stores
@@ -166,11 +267,12 @@ __global__ void k_pair_count(XPtrs xsp, const int32_t* __restrict__ start, const
 // writes the rest directly.  40 KiB per block.
 constexpr int kEmitStage = 1280;
 
-template <typename T, int D>
+template <typename T, int D, bool Stored>
 __global__ __launch_bounds__(256) void k_pair_emit(XPtrs xsp, const int32_t* __restrict__ start,
                                                    const int32_t* __restrict__ skey, const int32_t* __restrict__ id,
                                                    int64_t n, CellGeom g, T dist2max, const int64_t* __restrict__ off,
-                                                   int32_t* __restrict__ pairs, int64_t capacity) {
+                                                   int32_t* __restrict__ pairs, int64_t capacity,
+                                                   const uint64_t* __restrict__ ent, const int32_t* __restrict__ nent) {
   __shared__ int2 stage[kBlock / 64][kEmitStage];
   const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
   const int64_t s0 = int64_t(blockIdx.x) * kBlock + 64 * w;
@@ -182,22 +284,33 @@ __global__ __launch_bounds__(256) void k_pair_emit(XPtrs xsp, const int32_t* __r
   int2* out = reinterpret_cast<int2*>(pairs);
   if (s < n) {
     int64_t o = off[s];
-    const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
-                     static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
-    const T xa0 = x[0][s], xa1 = x[1][s], xa2 = D == 3 ? x[2][s] : T(0);
-    const Rows<D> r = stencil_rows<D>(g, start, skey[s]);
     const int32_t is = id[s] + 1;
+    // the neighbours in window [w0, w0 + 32) marked by mk, in ascending order
+    auto emit = [&](int32_t w0, uint32_t mk) {
+      for (; mk != 0 && o < capacity; mk &= mk - 1, ++o) {
+        const int32_t it = id[w0 + __builtin_ctz(mk)] + 1;
+        const int2 pr = is < it ? make_int2(is, it) : make_int2(it, is);
+        if (o - o0 < kEmitStage) st[o - o0] = pr;
+        else out[o] = pr;
+      }
+    };
+    const int ne = Stored ? nent[s] : -1;
+    if (ne >= 0) {
+      for (int e = 0; e < ne && o < capacity; ++e) {
+        const uint64_t en = ent[int64_t(e) * n + s];
+        emit(int32_t(en >> 32), uint32_t(en));
+      }
+    } else {
+      const T* x[3] = {static_cast<const T*>(xsp.p[0]), static_cast<const T*>(xsp.p[1]),
+                       static_cast<const T*>(xsp.p[D == 3 ? 2 : 1])};
+      const T xa0 = x[0][s], xa1 = x[1][s], xa2 = D == 3 ? x[2][s] : T(0);
+      const Rows<D> r = stencil_rows<D>(g, start, skey[s]);
 #pragma unroll
-    for (int k = 0; k < Rows<D>::N; ++k) {
-      const int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
-      for (int32_t w0 = b; w0 < r.e[k] && o < capacity; w0 += 32) {
-        const int32_t hi = r.e[k] < w0 + 32 ? r.e[k] : w0 + 32;
-        uint32_t mk = pair_window<T, D>(x, xa0, xa1, xa2, w0, hi, dist2max);
-        for (; mk != 0 && o < capacity; mk &= mk - 1, ++o) {
-          const int32_t it = id[w0 + __builtin_ctz(mk)] + 1;
-          const int2 pr = is < it ? make_int2(is, it) : make_int2(it, is);
-          if (o - o0 < kEmitStage) st[o - o0] = pr;
-          else out[o] = pr;
+      for (int k = 0; k < Rows<D>::N; ++k) {
+        const int32_t b = r.b[k] > int32_t(s) + 1 ? r.b[k] : int32_t(s) + 1;
+        for (int32_t w0 = b; w0 < r.e[k] && o < capacity; w0 += 32) {
+          const int32_t hi = r.e[k] < w0 + 32 ? r.e[k] : w0 + 32;
+          emit(w0, pair_window<T, D>(x, xa0, xa1, xa2, w0, hi, dist2max));
         }
       }
     }
@@ -312,12 +425,17 @@ void SortBuffers::release() {
   scan_tmp = nullptr;
 }
 
-void PairBuffers::alloc(int64_t n_) {
+void PairBuffers::alloc(int64_t n_, int entcap_) {
   release();
   n = n_;
   GCL_HIP(hipMalloc(&cnt, sizeof(int32_t) * size_t(n + 1)));
   GCL_HIP(hipMalloc(&off, sizeof(int64_t) * size_t(n + 1)));
   GCL_HIP(hipMalloc(&scan_tmp, sizeof(int64_t) * scan_tmp_elems(n)));
+  entcap = entcap_;
+  if (entcap > 0) {
+    GCL_HIP(hipMalloc(&ent, sizeof(uint64_t) * size_t(entcap) * size_t(n)));
+    GCL_HIP(hipMalloc(&nent, sizeof(int32_t) * size_t(n)));
+  }
 }
 
 void PairBuffers::set_capacity(int64_t cap, hipStream_t s) {
@@ -337,12 +455,15 @@ void PairBuffers::zero_from(int64_t from, hipStream_t s) {
 }
 
 void PairBuffers::release() {
-  for (void* p : {(void*)cnt, (void*)off, (void*)scan_tmp, (void*)pairs})
+  for (void* p : {(void*)cnt, (void*)off, (void*)scan_tmp, (void*)pairs, (void*)ent, (void*)nent})
     if (p) (void)hipFree(p);
   cnt = nullptr;
   off = nullptr;
   scan_tmp = nullptr;
   pairs = nullptr;
+  ent = nullptr;
+  nent = nullptr;
+  entcap = 0;
   capacity = 0;
 }
 
@@ -369,15 +490,21 @@ void gather_positions(int dim, int prec, const void* const* x, void* const* xs, 
 void pair_count(const CellGeom& g, int prec, const void* const* xs, const int32_t* start, const int32_t* skey,
                 int64_t n, double dist, PairBuffers& pb, hipStream_t s) {
   XPtrs xsp{{xs[0], xs[1], g.dim == 3 ? xs[2] : xs[1]}};
+  const bool store = g.dim == 2 && pb.entcap > 0;
+  auto run = [&](auto d2) {
+    using T = decltype(d2);
+    if (g.dim == 3)
+      k_pair_count<T, 3><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
+    else if (store)
+      k_pair_count2<T><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt, pb.ent, pb.nent, pb.entcap);
+    else
+      k_pair_count<T, 2><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
+  };
   if (prec == 32) {
     float d = float(dist);
-    float d2 = d * d;
-    if (g.dim == 2) k_pair_count<float, 2><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
-    else k_pair_count<float, 3><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
+    run(d * d);
   } else {
-    double d2 = dist * dist;
-    if (g.dim == 2) k_pair_count<double, 2><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
-    else k_pair_count<double, 3><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, n, g, d2, pb.cnt);
+    run(dist * dist);
   }
   GCL_HIP(hipGetLastError());
   exclusive_scan_i64(pb.cnt, pb.off, n, pb.scan_tmp, nullptr, s);
@@ -390,9 +517,12 @@ void launch_emit(bool wave, const XPtrs& xsp, const int32_t* start, const int32_
   if (wave)
     k_pair_emit_wave<T, D><<<grid_for(n, kBlock / 64), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off,
                                                                        pb.pairs, pb.capacity);
+  else if (D == 2 && pb.entcap > 0)
+    k_pair_emit<T, D, true><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
+                                                           pb.capacity, pb.ent, pb.nent);
   else
-    k_pair_emit<T, D><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
-                                                     pb.capacity);
+    k_pair_emit<T, D, false><<<grid_for(n), kBlock, 0, s>>>(xsp, start, skey, id, n, g, d2, pb.off, pb.pairs,
+                                                            pb.capacity, nullptr, nullptr);
 }
 }  // namespace
 

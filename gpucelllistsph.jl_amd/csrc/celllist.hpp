@@ -101,7 +101,12 @@ struct PairBuffers {
   int64_t* scan_tmp = nullptr;
   int32_t* pairs = nullptr;  // 2 * capacity
   int64_t capacity = 0;
-  void alloc(int64_t n_);
+  // 2D: the neighbour windows the count pass found, read back by the emit
+  // pass (entcap per item; 0: the emit pass tests the candidates again)
+  uint64_t* ent = nullptr;  // entcap * n
+  int32_t* nent = nullptr;  // n
+  int entcap = 0;
+  void alloc(int64_t n_, int entcap_ = 0);
   void set_capacity(int64_t cap, hipStream_t s);  // zero-filled
   // Zeroes the entries [from, capacity).  A search with fill-zero leaves the
   // pairs in [0, pairsn) and zeros after them: the emit writes the front, so
@@ -150,6 +155,36 @@ __device__ __forceinline__ Rows<D> stencil_rows(const CellGeom& g, const int32_t
   return r;
 }
 
+// The union of the stencil rows of `span` consecutive particles from a0 (a
+// block or a wave) in one plane (2D: the only plane; 3D: the plane dz ∈ {-1,
+// 0, 1}).  The particles a0..a1 occupy one contiguous range of cells [c0, c1],
+// so for each row offset k the cells [c0 + o - 1, c1 + o + 1], o = (k-1)·gx +
+// dz·gx·gy, hold every candidate of that row of every one of them: three
+// contiguous ranges of sorted particles starting at ub[k], laid out one after
+// the other from LDS slot uo[k].
+struct RowUnion {
+  int32_t ub[3], uo[4];
+};
+
+__device__ __forceinline__ void row_union(RowUnion& u, const int32_t* start, const int32_t* skey, const CellGeom& G,
+                                          int64_t n, int64_t a0, int dz = 0, int span = kBlock) {
+  const int64_t a1 = (a0 + span < n ? a0 + span : n) - 1;
+  const int64_t c0 = skey[a0], c1 = skey[a1];
+  const int64_t gx = G.grid[0], nc = G.ncells();
+  const int64_t zo = int64_t(dz) * gx * G.grid[1];
+  int32_t o = 0;
+  for (int k = 0; k < 3; ++k) {
+    int64_t lo = c0 + (k - 1) * gx + zo - 1, hi = c1 + (k - 1) * gx + zo + 1;
+    lo = lo < 0 ? 0 : (lo > nc - 1 ? nc - 1 : lo);
+    hi = hi < 0 ? 0 : (hi > nc - 1 ? nc - 1 : hi);
+    const int32_t b = start[lo], e = start[hi + 1];
+    u.ub[k] = b;
+    u.uo[k] = o;
+    o += e > b ? e - b : 0;
+  }
+  u.uo[3] = o;
+}
+
 // Bins positions `x` (D pointers, input order, element type double or float)
 // into cells, and writes the sorted order (perm, skey, start).  The order
 // inside a cell is by input index, so the result is deterministic.
@@ -180,6 +215,10 @@ void pair_count(const CellGeom& g, int prec, const void* const* xs, const int32_
 // 1.05 ms at 71 pairs per point.)  The environment variable
 // GCLSPH_PAIR_EMIT=lane|wave forces one of them.
 constexpr int64_t kEmitWaveMin = 64;
+// Neighbour windows per item the 2D count pass keeps for the emit pass (an
+// item of the 2D dam break at dist = H has 2-4; an item with more is tested
+// again by the emit pass).  48 B per item.
+constexpr int kPairEntries2D = 6;
 void pair_emit(const CellGeom& g, int prec, const void* const* xs, const int32_t* start, const int32_t* skey,
                const int32_t* id, int64_t n, double dist, int64_t total, PairBuffers& pb, hipStream_t s);
 

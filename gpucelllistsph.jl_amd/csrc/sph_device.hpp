@@ -387,36 +387,6 @@ __device__ __forceinline__ typename Records<D, R>::N* rec_n(void* base) {
   return static_cast<typename Records<D, R>::N*>(base);
 }
 
-// The union of the stencil rows of `span` consecutive particles from a0 (a
-// block or a wave) in one plane (2D: the only plane; 3D: the plane dz ∈ {-1,
-// 0, 1}).  The particles a0..a1 occupy one contiguous range of cells [c0, c1],
-// so for each row offset k the cells [c0 + o - 1, c1 + o + 1], o = (k-1)·gx +
-// dz·gx·gy, hold every candidate of that row of every one of them: three
-// contiguous ranges of sorted particles starting at ub[k], laid out one after
-// the other from LDS slot uo[k].
-struct RowUnion {
-  int32_t ub[3], uo[4];
-};
-
-__device__ __forceinline__ void row_union(RowUnion& u, const int32_t* start, const int32_t* skey, const CellGeom& G,
-                                          int64_t n, int64_t a0, int dz = 0, int span = kBlock) {
-  const int64_t a1 = (a0 + span < n ? a0 + span : n) - 1;
-  const int64_t c0 = skey[a0], c1 = skey[a1];
-  const int64_t gx = G.grid[0], nc = G.ncells();
-  const int64_t zo = int64_t(dz) * gx * G.grid[1];
-  int32_t o = 0;
-  for (int k = 0; k < 3; ++k) {
-    int64_t lo = c0 + (k - 1) * gx + zo - 1, hi = c1 + (k - 1) * gx + zo + 1;
-    lo = lo < 0 ? 0 : (lo > nc - 1 ? nc - 1 : lo);
-    hi = hi < 0 ? 0 : (hi > nc - 1 ? nc - 1 : hi);
-    const int32_t b = start[lo], e = start[hi + 1];
-    u.ub[k] = b;
-    u.uo[k] = o;
-    o += e > b ? e - b : 0;
-  }
-  u.uo[3] = o;
-}
-
 // Neighbour-list partners b of particle a in a fixed order (stencil rows in
 // order, ascending sorted index inside a row).  nc >= 0: the nc stored entries
 // of k_nmask, read one ahead so the next load overlaps the current window's

@@ -176,6 +176,38 @@ def test_pair_emit_variants_agree(G, O, S, monkeypatch, dim, dist):
     assert np.array_equal(as_set(bufs[1][: o.info()["pairsn"]]), as_set(o.neighborlistview()))
 
 
+@pytest.mark.parametrize("cs", [0.1, 0.3])
+def test_stored_windows_with_large_cells(G, O, S, monkeypatch, cs):
+    """The 2D count pass keeps each point's neighbour windows for the emit pass
+    (at most 6; a point with more is tested again by the emit) and stages the
+    candidates of a wave's stencil rows in LDS when they fit.  Points sorted by
+    x in cells 5-15 times dist: a wave's rows hold far more candidates than the
+    LDS slots (the global-memory path of both passes), while a point's
+    neighbours lie in a run of x-neighbours per cell, so most points' windows
+    fit the store and some do not.  Lane and wave emits write the same buffer,
+    equal as a set to the oracle's, after update! and partialupdate!."""
+    pts = S.uniform_points(31, 30000, 2)
+    order = np.argsort(pts[0], kind="stable")
+    pts = tuple(p[order] for p in pts)
+    dist = 0.02
+    o = O.CellList(pts, cs, dist)
+    bufs = []
+    for mode in ("lane", "wave"):
+        monkeypatch.setenv("GCLSPH_PAIR_EMIT", mode)
+        g = G.GPUCellList(pts, cs, dist)
+        G.update_(g)
+        assert g.pairsn == o.info()["pairsn"]
+        bufs.append(G.neighborlist(g))
+        g.points = S.jitter(pts, 5, 1, sigma=0.002)
+        G.partialupdate_(g)
+        bufs.append(G.neighborlist(g)[: g.pairsn])
+    assert np.array_equal(bufs[0], bufs[2]) and np.array_equal(bufs[1], bufs[3])
+    assert np.array_equal(as_set(bufs[0][: o.info()["pairsn"]]), as_set(o.neighborlistview()))
+    o.set_points(S.jitter(pts, 5, 1, sigma=0.002))
+    o.partialupdate()
+    assert np.array_equal(as_set(bufs[1]), as_set(o.neighborlistview()))
+
+
 def test_points_outside_min_max_clamp(G, O, S):
     pts = S.uniform_points(8, 20000, 2)
     kw = dict(MIN=(0.25, 0.2), MAX=(0.7, 0.9))
