@@ -225,6 +225,9 @@ def main():
     ap.add_argument("--cl-points", choices=("dambreak", "uniform200k"), default="dambreak",
                     help="points of --scene celllist: the 4 M-particle dam break, or the 200,000 uniform points "
                          "of the reference's own update!/partialupdate! benchmark")
+    ap.add_argument("--walls", choices=("fixture", "minus1"), default="fixture",
+                    help="wall typing of the 2D dam break: the reference fixtures' (inner layer 1, outer 0) "
+                         "or ptype -1 for every wall particle")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "pass2_f32split_traffic.txt"))
     ap.add_argument("--calibrate-copy", type=float, default=0.0,
                     help="copy a buffer of this many GiB three times and exit (HBM counter calibration)")
@@ -271,7 +274,7 @@ def main():
                                      "particles": cl.n, "pairs": cl.pairsn}}))
         return
     if args.scene == "celllist":
-        sc = S.config3(dx=args.dx)
+        sc = S.config3(dx=args.dx, walls=-1 if args.walls == "minus1" else "fixture")
         cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
         for _ in range(args.warmup):
             G.update_(cl)
@@ -299,7 +302,7 @@ def main():
                       "stage_ms_per_step": {k: t[0] / args.steps for k, t in v["timing"].items() if t[1] > 0}}
                for prec, v in r.items()}}))
         return
-    scene = S.config3(dx=args.dx)
+    scene = S.config3(dx=args.dx, walls=-1 if args.walls == "minus1" else "fixture")
     n = scene.n
     log(f"scene: {n} particles; FP32-split leg")
     res32 = run_gpu(scene, "f32split", args.steps, args.warmup)
@@ -333,7 +336,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"2D dam break dx={args.dx}: 4.0x4.0 m column ({int(scene.info['fluid'])} fluid) in a "
-                        f"16x8 m tank ({int(scene.info['boundary'])} wall), WendlandC2, dist=H, adaptive CFL=0.2",
+                        f"16x8 m tank ({int(scene.info['boundary'])} wall, "
+                        f"{'ptype -1' if args.walls == 'minus1' else 'fixture typing: inner layer 1, outer 0'}), "
+                        "WendlandC2, dist=H, adaptive CFL=0.2",
             "particles": n,
             "rebuilds_in_timed_steps": res32["updaten"],
             "precision": "FP32-split",

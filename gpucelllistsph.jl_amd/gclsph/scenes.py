@@ -266,10 +266,12 @@ def config2(dx: float = 0.01) -> Scene:
     return sc
 
 
-def config3(dx: float = 0.002, fluid: float = 4.0) -> Scene:
-    """Large 2D dam break: 4 x 4 m column at dx=0.002 (4,000,000 fluid) in a 16 x 8 m tank,
-    walls typed as in the reference's fixtures (inner layer 1, outer layers 0)."""
-    return dam_break_2d(dx, fluid, fluid, 16.0, 8.0, name=f"dam-break-2d dx={dx} ({fluid}x{fluid} m column)")
+def config3(dx: float = 0.002, fluid: float = 4.0, walls="fixture") -> Scene:
+    """Large 2D dam break: 4 x 4 m column at dx=0.002 (4,000,000 fluid) in a 16 x 8 m tank.
+    walls: -1 (every wall particle ptype -1, as config 0's walls) or "fixture" (typed as in the
+    reference's fixtures: inner layer 1, outer layers 0)."""
+    return dam_break_2d(dx, fluid, fluid, 16.0, 8.0, boundary_ptype=walls,
+                        name=f"dam-break-2d dx={dx} ({fluid}x{fluid} m column, walls {walls})")
 
 
 def config4(seed: int = 4, count: int = 64) -> list:
