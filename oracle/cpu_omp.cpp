@@ -54,7 +54,8 @@ void omp_rebuild(OmpSph& S) {
     int64_t c = 0, mul = 1;
     for (int k = 0; k < D; ++k) {
       double q = (S.x[k][i] - S.off[k]) * hinv[k];
-      double cl = std::min(std::max(q, 1.0), double(S.grid[k]));
+      double cl = q >= 1.0 ? q : 1.0;  // min(max(q, 1), grid); NaN -> cell 1 as in the oracle
+      cl = cl <= double(S.grid[k]) ? cl : double(S.grid[k]);
       c += (int64_t(std::ceil(cl)) - 1) * mul;
       mul *= S.grid[k];
     }

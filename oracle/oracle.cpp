@@ -88,7 +88,10 @@ void cellmap(CellList<T>& c, bool use_off64) {
         T pt = (c.x[k][i] - c.off[k]) * hinv[k];
         p = double(pt);
       }
-      double cl = std::min(std::max(p, 1.0), double(c.grid[k]));
+      // min(max(p, 1), grid); a NaN coordinate goes to cell 1, as the GPU
+      // library documents (the reference leaves it undefined: ceil(Int32, NaN))
+      double cl = p >= 1.0 ? p : 1.0;
+      cl = cl <= double(c.grid[k]) ? cl : double(c.grid[k]);
       idx[k] = int32_t(std::ceil(cl));
       c.pcell[k][i] = idx[k];
     }

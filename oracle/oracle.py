@@ -15,12 +15,14 @@ from typing import Optional, Sequence, Tuple
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# GCLSPH_ORACLE_LIB / GCLSPH_ORACLE_OMP_LIB: other builds of the same sources
+# (the sanitizer builds of `make sanitize`)
+LIB_PATH = os.environ.get("GCLSPH_ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 _lib = None
 
 
-OMP_PATH = os.path.join(HERE, "_build", "libcpuomp.so")
+OMP_PATH = os.environ.get("GCLSPH_ORACLE_OMP_LIB") or os.path.join(HERE, "_build", "libcpuomp.so")
 _omp = None
 
 
