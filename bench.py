@@ -182,7 +182,7 @@ def read_profile(path: str) -> dict:
     return vals
 
 
-def cpu_baseline(scene, seconds_budget: float = 15.0, max_steps: int = 20):
+def cpu_baseline(scene, seconds_budget: float = 15.0, max_steps: int = 30):
     """The reference's step on all host cores: oracle/cpu_omp.cpp, an OpenMP
     per-particle gather restating the reference's per-pair arithmetic in FP64
     (checked against the serial oracle in tests/).  Sample: consecutive
