@@ -1067,7 +1067,7 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
     }
     g->upload_points(pts);
     g->sb.alloc(n, g->G.ncells());
-    g->pb.alloc(n, dim == 2 ? kPairEntries2D : 0);
+    g->pb.alloc(n, dim == 2 && n <= kPairEntriesMaxPoints ? kPairEntries2D : 0);
     // cellmap! with the Float64 MIN (:55), then fillcells_naive! (:66)
     bin_sort(g->G, prec, g->xo, n, true, g->sb, nullptr, g->stream);
     int32_t maxc = g->read_maxcell();

@@ -217,8 +217,10 @@ void pair_count(const CellGeom& g, int prec, const void* const* xs, const int32_
 constexpr int64_t kEmitWaveMin = 64;
 // Neighbour windows per item the 2D count pass keeps for the emit pass (an
 // item of the 2D dam break at dist = H has 2-4; an item with more is tested
-// again by the emit pass).  48 B per item.
+// again by the emit pass).  48 B per item, kept for cell lists of up to
+// 2^27 points (6 GiB); larger ones test the candidates again.
 constexpr int kPairEntries2D = 6;
+constexpr int64_t kPairEntriesMaxPoints = int64_t(1) << 27;
 void pair_emit(const CellGeom& g, int prec, const void* const* xs, const int32_t* start, const int32_t* skey,
                const int32_t* id, int64_t n, double dist, int64_t total, PairBuffers& pb, hipStream_t s);
 
