@@ -328,6 +328,9 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        # untimed steps actually run before the timed call: at least TUNE_STEPS
+        # (the autotuner's), then the 2-step call that captures the step graph
+        "warmup_steps_run": res32["warmup_steps"],
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
