@@ -414,6 +414,40 @@ def test_pass_variants_bit_identical(G, monkeypatch, prec):
             assert np.array_equal(np.asarray(a), np.asarray(b)), v
 
 
+@pytest.mark.parametrize("n", [1, 2, 63, 65, 257, 1000])
+def test_tiny_and_ragged_problems(G, O, S, n):
+    """Problems of 1 and 2 particles and counts that leave the last wave or
+    block of every kernel part-empty (63, 65, 257, 1000): fluid on a jittered
+    lattice (plus two ptype -1 particles from n = 63 on), 12 adaptive steps in
+    several calls (eager single steps and captured multi-step calls), FP64
+    against the oracle at the golden tolerances and FP32-split within 1e-5·dx
+    of it."""
+    dx = 0.02
+    p, dist = S.standard_params(dx, 2)
+    side = int(math.ceil(math.sqrt(n)))
+    k = np.arange(n)
+    x = 0.1 + (k % side) * dx + 0.1 * dx * (S.uniform01(7, n, 0) - 0.5)
+    y = 0.1 + (k // side) * dx + 0.1 * dx * (S.uniform01(7, n, 1) - 0.5)
+    ptype = np.full(n, 2, np.int32)
+    if n >= 63:  # two wall particles 3·dx below the first row: in the lists, outside the wall force's reach
+        ptype[:2] = -1
+        y[:2] = 0.1 - 3 * dx
+    rho = np.full(n, p.rho0) * (1 + 1e-3 * S.uniform01(7, n, 2))
+    sc = S.Scene(2, (x, y), rho, ptype, p, dist, (dist, dist))
+    cl, oc, p64, ref = _pair(G, O, sc)
+    cl32 = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+    p32 = G.SPHProblem.from_params(cl32, sc.params, sc.rho, sc.ptype, precision="f32split")
+    for steps in (1, 1, 4, 6):
+        G.stepsolve_(p64, steps, timestepping=True)
+        G.stepsolve_(p32, steps, timestepping=True)
+        ref.stepsolve(steps, True)
+    got = gpu_fields(G, p64)
+    for f in ("x", "v", "rho", "acc"):
+        assert rel_err(got[f], ref.get(f)) <= TOL64[f], f
+    assert np.abs(np.array(G.get_points(p32)) - ref.get("x")).max() < 1e-5 * dx
+    assert np.abs(G.get_density(p32) / ref.get("rho") - 1).max() < 1e-6
+
+
 def test_wall_force_fp32_and_fp64_branches(G, O, S):
     """FP32-split evaluates the wall force in FP32 while q = r0/(r + η²) < 32 and
     in FP64 for closer pairs.  One fluid particle 0.02·dx above a wall particle
