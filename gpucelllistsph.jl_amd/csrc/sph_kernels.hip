@@ -309,10 +309,22 @@ __device__ __forceinline__ void nmask2_perm(const SphBufs<R>& B, const SphConst&
   if (total <= kNmaskWave) {  // uniform over the wave
     double2* wl = lds[w];
     const int32_t d0 = u.ub[0] - u.uo[0], d1 = u.ub[1] - u.uo[1], d2 = u.ub[2] - u.uo[2];
-    for (int32_t i = lane; i < total; i += 64) {
-      const int32_t b = i + (i >= u.uo[2] ? d2 : (i >= u.uo[1] ? d1 : d0));
-      const int32_t src = perm[b];
-      wl[i] = make_double2(XO[0][src], XO[1][src]);
+    // four slots per lane per trip, so four dependent load chains (perm, then
+    // the positions) are in flight together; a slot past the union repeats
+    // the last valid one's loads and is not written
+    for (int32_t i0 = lane; i0 < total; i0 += 4 * 64) {
+      int32_t src[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int32_t i = i0 + 64 * q < total ? i0 + 64 * q : total - 1;
+        src[q] = perm[i + (i >= u.uo[2] ? d2 : (i >= u.uo[1] ? d1 : d0))];
+      }
+      double2 v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = make_double2(XO[0][src[q]], XO[1][src[q]]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (i0 + 64 * q < total) wl[i0 + 64 * q] = v[q];
     }
     // the wave's LDS operations complete in issue order; the fence keeps the
     // compiler from moving the reads below ahead of these writes
