@@ -31,6 +31,10 @@ import gclsph as G  # noqa: E402
 from gclsph import scenes as S  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+# MI355X VALU issue peak: 256 CUs x 4 SIMDs, a wave64 instruction every 4
+# cycles per SIMD, at the 2.4 GHz peak engine clock (1 wave64 instruction per
+# CU per cycle; the FP32 vector peak without packed math, 78.6 TFLOP/s in FMA)
+VALU_PEAK_GIPS = 256 * 2.4
 # The first call of more than 6 steps on a problem or batch times the pass
 # variants on its first 6 steps (4 code-generation variants, then 2 more block
 # sizes); a warmup call of at least 7 steps keeps that tuning out of the timed
@@ -167,14 +171,16 @@ def run_ensemble(steps: int, warmup: int):
 def read_profile(path: str) -> dict:
     """`key: value` lines of the committed rocprofv3 PMC summary of the dominant
     kernel: HBM bytes per dispatch (FETCH_SIZE / WRITE_SIZE scaled by the
-    copy-kernel calibration) and the VALU busy fraction."""
+    copy-kernel calibration), the VALU busy fraction and the wave64 VALU
+    instructions per dispatch (SQ_INSTS_VALU)."""
     vals = {}
     if not os.path.exists(path):
         return vals
     with open(path) as fh:
         for line in fh:
             k, sep, v = line.partition(":")
-            if sep and k.strip() in ("traffic_bytes_per_dispatch", "valu_busy_per_dispatch"):
+            if sep and k.strip() in ("traffic_bytes_per_dispatch", "valu_busy_per_dispatch",
+                                     "valu_instructions_per_dispatch"):
                 try:
                     vals[k.strip()] = float(v.split()[0])
                 except (ValueError, IndexError):
@@ -364,6 +370,13 @@ def main():
             "traffic": traffic,
             "bytes_per_particle": bytes_pp[dom],
             "valu_busy": prof.get("valu_busy_per_dispatch"),
+            # the roof that binds the pair passes: VALU issue (instructions per
+            # launch from the committed SQ counters / the same measured time)
+            "valu": None if "valu_instructions_per_dispatch" not in prof else {
+                "instructions_per_launch": prof["valu_instructions_per_dispatch"],
+                "achieved": prof["valu_instructions_per_dispatch"] / (dom_ms * 1e-3) / 1e9,
+                "peak": VALU_PEAK_GIPS, "unit": "G wave64 VALU instructions/s",
+                "frac": prof["valu_instructions_per_dispatch"] / (dom_ms * 1e-3) / 1e9 / VALU_PEAK_GIPS},
             "note": "pair passes are co-limited by vector-L1 access rate and VALU issue, not HBM: "
                     "achieved = algorithmic bytes / measured kernel time; "
                     "valu_busy (share of SIMD cycles issuing VALU) and traffic come from the committed PMC profile",
