@@ -336,10 +336,11 @@ __global__ __launch_bounds__(256) void k_pair_emit(XPtrs xsp, const int32_t* __r
     const int32_t it = id[q.y] + 1;
     return q.x < it ? make_int2(q.x, it) : make_int2(it, q.x);
   };
-  // two pairs per lane in 16-byte stores from the first even output slot;
-  // an odd first slot and an odd last pair go out on their own
+  // two pairs per lane in 16-byte stores from the first 16-byte aligned output
+  // slot (pair buffers are 8-byte aligned); a first slot off that alignment
+  // and an odd last pair go out on their own
   const int lane = int(threadIdx.x & 63);
-  const int64_t h = (o0 & 1) && ns > 0 ? 1 : 0;
+  const int64_t h = (reinterpret_cast<uintptr_t>(out + o0) & 15) != 0 && ns > 0 ? 1 : 0;
   if (h && lane == 0) out[o0] = pair_at(0);
   const int64_t body = (ns - h) / 2;
   int4* out4 = reinterpret_cast<int4*>(out + o0 + h);

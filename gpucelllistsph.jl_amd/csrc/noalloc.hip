@@ -100,6 +100,8 @@ extern "C" int gcl_neighborlist_noalloc(int dim, int64_t n, const double* const*
     for (int k = 0; k < dim; ++k) GCL_CHECK(pts_dev[k] != nullptr, kErrArgument, "points component is NULL");
     GCL_CHECK(capacity >= 0 && (capacity == 0 || pairs_dev != nullptr), kErrArgument,
               "pair buffer is NULL or its capacity negative");
+    GCL_CHECK(reinterpret_cast<uintptr_t>(pairs_dev) % 8 == 0, kErrArgument,
+              "pair buffer must be 8-byte aligned (one (i, j) pair per 8 bytes)");
     GCL_CHECK(dist > 0 && std::isfinite(dist), kErrArgument, "dist must be positive and finite");
     // The reference constructor's grid with MIN and MAX given (src/celllist.jl:27-46):
     // a cell smaller than dist is raised to dist, offset MIN - |(MIN + √eps)·√eps|,

@@ -106,7 +106,8 @@ int gcl_maxpairs(gcl_t h, int64_t* out);
  * on the same Float64 points, 1-based (i, j) with i < j, with the cell grid
  * of the given MIN / MAX (host arrays of dim values; points outside clamp
  * into the edge cells).  pts_dev: dim DEVICE pointers to n Float64
- * coordinates.  pairs_dev: DEVICE buffer of 2*capacity int32; the first
+ * coordinates.  pairs_dev: DEVICE buffer of 2*capacity int32, 8-byte aligned
+ * (as hipMalloc returns it); the first
  * min(pairsn, capacity) pairs are written, entries after them are left as
  * they were.  pairsn (host): the total number of pairs.  stream: the
  * hipStream_t to run on (NULL: the null stream); the call returns when the

@@ -75,3 +75,5 @@ def test_noalloc_rejects_oversized_inputs_before_touching_a_device(G):
         G.neighborlist_noalloc([1, 2], 1000, 1e-4, 1e-4, (0, 0), (1, 1), 3, 8)
     with pytest.raises(ValueError, match="dimention"):
         G.neighborlist_noalloc([1], 10, 0.1, 0.1, (0,), (1,), 3, 8)
+    with pytest.raises(ValueError, match="8-byte aligned"):
+        G.neighborlist_noalloc([8, 16], 10, 0.1, 0.1, (0, 0), (1, 1), 4 + 8, 8)
