@@ -69,11 +69,12 @@ def test_argument_errors_before_device(G):
 def test_noalloc_rejects_oversized_inputs_before_touching_a_device(G):
     """gcl_neighborlist_noalloc checks its fixed maxima on the host first, so
     these errors need no GPU."""
+    # stand-in device addresses: never dereferenced, the checks come first
     with pytest.raises(ValueError, match="NOALLOC_MAX_POINTS"):
-        G.neighborlist_noalloc([1, 2], G.api.NOALLOC_MAX_POINTS + 1, 0.1, 0.1, (0, 0), (1, 1), 3, 8)
+        G.neighborlist_noalloc([8, 16], G.api.NOALLOC_MAX_POINTS + 1, 0.1, 0.1, (0, 0), (1, 1), 32, 8)
     with pytest.raises(ValueError, match="NOALLOC_MAX_CELLS"):
-        G.neighborlist_noalloc([1, 2], 1000, 1e-4, 1e-4, (0, 0), (1, 1), 3, 8)
+        G.neighborlist_noalloc([8, 16], 1000, 1e-4, 1e-4, (0, 0), (1, 1), 32, 8)
     with pytest.raises(ValueError, match="dimention"):
-        G.neighborlist_noalloc([1], 10, 0.1, 0.1, (0,), (1,), 3, 8)
+        G.neighborlist_noalloc([8], 10, 0.1, 0.1, (0,), (1,), 32, 8)
     with pytest.raises(ValueError, match="8-byte aligned"):
         G.neighborlist_noalloc([8, 16], 10, 0.1, 0.1, (0, 0), (1, 1), 4 + 8, 8)
