@@ -305,15 +305,17 @@ struct sph_s {
     B.PRH = dalloc<uint8_t>(int64_t(rh) * n);
   }
 
-  // Entry store of k_nmask: a particle needs at most one entry per 32-candidate
-  // window of its 3^D stencil rows, each row holding at most 3 * mppcell
-  // candidates.  GCLSPH_NMASK_CAP lowers the capacity (particles beyond it
-  // scan their stencil instead).  Reallocation drops the stored membership.
+  // Entry store of the membership: a particle needs at most one entry per
+  // window, 32-candidate windows of its 3^(D-1) stencil rows (each holding at
+  // most 3 * mppcell candidates), or in 3D FP32-split kWin3-candidate windows
+  // of its 27 stencil cells.  GCLSPH_NMASK_CAP lowers the capacity (particles
+  // beyond it scan their stencil instead).  Reallocation drops the stored
+  // membership.
   template <typename R>
   void ensure_nmask(SphBufs<R>& B) {
     const int64_t mpp = std::max<int64_t>({hctl.mppcell, sys->mppcell, 1});
-    const int64_t rows = dim == 2 ? 3 : 9;
-    int64_t need = rows * ((3 * mpp + 31) / 32);
+    const bool cells = dim == 3 && std::is_same<R, float>::value;  // cell_windows<3, R>()
+    int64_t need = cells ? 27 * ((mpp + kWin3 - 1) / kWin3) : (dim == 2 ? 3 : 9) * ((3 * mpp + 31) / 32);
     if (const char* env = std::getenv("GCLSPH_NMASK_CAP")) {
       const long v = std::strtol(env, nullptr, 10);
       if (v >= 0 && v < need) need = v;
@@ -442,6 +444,10 @@ struct sph_s {
     C.f_dpc_lambda = float(C.dpc_lambda);
     C.f_dpc_pmin = float(C.dpc_pmin);
     C.f_dpc_pmax = float(C.dpc_pmax);
+    for (int k = 0; k < 3; ++k) {
+      C.f_fcs[k] = k < dim ? float(sys->cs[k]) : 0.0f;
+      C.fcs[k] = double(C.f_fcs[k]);
+    }
     return C;
   }
 

@@ -1,6 +1,7 @@
 // SPH step: device state, constants and the host-side driver.
 #pragma once
 
+#include <type_traits>
 #include <vector>
 
 #include "celllist.hpp"
@@ -34,6 +35,8 @@ struct SphConst {
   float f_inv_rho0;  // 1/ρ0
   float f_st_turn;   // 1.5π/(2h) in turns: cos(1.5π r/2h) = cos(2π · r·f_st_turn)
   float f_dwr;       // dW/dr / r = f_dwr (1-u)^3: -20 dwnorm H⁻¹
+  double fcs[3];     // cell-frame spacing of the 3D FP32-split partner records: the
+  float f_fcs[3];    // cell sizes rounded to FP32 (exact in both copies)
 };
 
 // Switches of one problem.  The host sets the per-call ones; the per-step ones
@@ -54,6 +57,15 @@ struct StepFlags {
   int cspmn;        // CSPM every cspmn iterations of a call (0: never)
   int cspm_fire;    // CSPM runs after this step
 };
+
+// Candidates per 3D FP32-split membership window (one stencil cell; the
+// cell's offset code, 0..26, takes the five bits above the member bits).  The
+// other layouts (2D, 3D FP64) use 32-candidate windows of a stencil row.
+constexpr int kWin3 = 27;
+template <int D, typename R>
+constexpr bool cell_windows() {
+  return D == 3 && std::is_same<R, float>::value;
+}
 
 // Position arrays (both parities and the rebuild copy) carry this many zero
 // elements past n: the membership test reads candidates in groups of four and
@@ -122,11 +134,31 @@ struct SphBufs {
 // passes which layout the step uses).  The
 // pressure is not stored: it is the EOS of (rho, pt), recomputed with the same
 // function that fills PN / PH.
+//
+// In 3D the position is held as FP32 offsets from the origin of the
+// particle's cell in the current sort (its cell frame, cell_frame below): a
+// partner in the stencil cell at offset o from a's cell is at separation
+// (x_a - frame(c_a + o)) - xo_b, where the first term is a per-window constant
+// of a (the stored windows name their cell, so it is computed once per
+// window), and the record shrinks to 32 bytes: two 16-byte loads per partner
+// instead of three.  The offsets of a particle inside its cell are below one
+// cell size, so their FP32 rounding (2^-24 of a cell) is of the same order as
+// the rounding of the FP32 separation itself.  A particle outside the grid
+// (binned into an edge cell) sits further from its frame and loses
+// precision in proportion.  2D keeps FP64 positions: its record is two loads
+// either way.
 template <int D>
 struct alignas(16) PartnerN {
   double x[D];
   float v[D];
   float rho;  // ρ - ρ0
+  int32_t pt;
+};
+template <>
+struct alignas(16) PartnerN<3> {
+  float xo[3];  // x - cell frame
+  float v[3];
+  float rho;
   int32_t pt;
 };
 template <int D>
@@ -136,14 +168,21 @@ struct alignas(16) PartnerH {
   float rh;  // ρ½ - ρ0
   int32_t pt;
 };
-static_assert(sizeof(PartnerN<2>) == 32 && sizeof(PartnerN<3>) == 48, "partner record sizes");
+template <>
+struct alignas(16) PartnerH<3> {
+  float xo[3];  // x½ - cell frame
+  float vh[3];
+  float rh;
+  int32_t pt;
+};
+static_assert(sizeof(PartnerN<2>) == 32 && sizeof(PartnerN<3>) == 32, "partner record sizes");
 template <int D>
 struct alignas(16) PartnerHX {
   PartnerH<D> h;
   float dn[D];  // x½ - xⁿ
 };
-static_assert(sizeof(PartnerH<2>) == 32 && sizeof(PartnerH<3>) == 48, "partner record sizes");
-static_assert(sizeof(PartnerHX<2>) == 48 && sizeof(PartnerHX<3>) == 64, "partner record sizes");
+static_assert(sizeof(PartnerH<2>) == 32 && sizeof(PartnerH<3>) == 32, "partner record sizes");
+static_assert(sizeof(PartnerHX<2>) == 48 && sizeof(PartnerHX<3>) == 48, "partner record sizes");
 
 // FP64 path: the pressure is stored (its EOS costs an FP64 division and a
 // power per partner), and pass 2's record always carries xⁿ (an FP64-exact

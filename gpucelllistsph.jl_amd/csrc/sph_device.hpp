@@ -387,14 +387,123 @@ __device__ __forceinline__ typename Records<D, R>::N* rec_n(void* base) {
   return static_cast<typename Records<D, R>::N*>(base);
 }
 
-// Neighbour-list partners b of particle a in a fixed order (stencil rows in
-// order, ascending sorted index inside a row).  nc >= 0: the nc stored entries
-// of k_nmask, read one ahead so the next load overlaps the current window's
-// pair bodies.  nc < 0: the windows are computed here from the membership
-// positions XM.  Each lane runs its own sequence, so all lanes of a wave
-// execute the (long) pair body together.
+// ---- 3D cell frames and per-cell membership windows
+// 3D membership entries hold one window of at most kWin3 candidates inside
+// one stencil cell: the member bits in the low kWin3 bits of the low word and
+// above them the cell's offset o from the particle's own cell, coded
+// (ox+1) + 3(oy+1) + 9(oz+1); the high word is the window base.  The
+// FP32-split 3D pair kernels turn that code into the per-window separation
+// constants of their cell frames (partner records in 3D hold positions as
+// offsets from their cell's frame origin).
+constexpr uint32_t kWinMask3 = (1u << kWin3) - 1u;
+__device__ __forceinline__ uint32_t win_code3(int ox, int oy, int oz) {
+  return uint32_t(ox + 1) + 3u * uint32_t(oy + 1) + 9u * uint32_t(oz + 1);
+}
+
+// Origin of the cell frame along axis d of grid coordinate i (0-based).
+// Every writer and reader of a 3D record evaluates this same expression.
+__device__ __forceinline__ double frame_origin(const SphConst& C, const CellGeom& G, int d, int64_t i) {
+#pragma clang fp contract(off)
+  return G.off64[d] + double(i) * C.fcs[d];
+}
+
+// Frame origins of sorted cell c.
+__device__ __forceinline__ void cell_frame3(const SphConst& C, const CellGeom& G, int32_t c, double fr[3]) {
+  int64_t ix, iy, iz;
+  cell_xyz<3>(G, c, ix, iy, iz);
+  fr[0] = frame_origin(C, G, 0, ix);
+  fr[1] = frame_origin(C, G, 1, iy);
+  fr[2] = frame_origin(C, G, 2, iz);
+}
+
+// Separation constants of a particle at xa in sorted cell c: km[d] = xa_d -
+// frame_d(c_d - 1), rounded once, and the frame cell sizes cf (FP32-exact).
+// For a partner b in the stencil cell at offset o, xa - xb = K - xo_b with
+// K = km - (o + 1)·cf, one FMA whose product is exact.
+struct FrameK3 {
+  float km[3];
+  float cf[3];
+};
+__device__ __forceinline__ FrameK3 frame_k3(const SphConst& C, const CellGeom& G, int32_t c, const double* xa) {
+  int64_t i[3];
+  cell_xyz<3>(G, c, i[0], i[1], i[2]);
+  FrameK3 t;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    t.km[d] = float(xa[d] - frame_origin(C, G, d, i[d] - 1));
+    t.cf[d] = C.f_fcs[d];
+  }
+  return t;
+}
+// Window constants of one partner (3D FP32-split); an empty type elsewhere.
+struct Win3 {
+  float k[3];
+};
+struct NoWin {};
+
+// Position fields of the FP32-split partner records: FP64 in 2D; in 3D the
+// FP32 offset from the frame origins fr of the particle's sorted cell.
+template <int D>
+__device__ __forceinline__ void put_x(PartnerN<D>& r, int d, double x, const double*) {
+  r.x[d] = x;
+}
+__device__ __forceinline__ void put_x(PartnerN<3>& r, int d, double x, const double* fr) {
+  r.xo[d] = float(x - fr[d]);
+}
+template <int D>
+__device__ __forceinline__ void put_x(PartnerN64<D>& r, int d, double x, const double*) {
+  r.x[d] = x;
+}
+template <int D>
+__device__ __forceinline__ void put_x(PartnerH<D>& r, int d, double x, const double*) {
+  r.xh[d] = x;
+}
+__device__ __forceinline__ void put_x(PartnerH<3>& r, int d, double x, const double* fr) {
+  r.xo[d] = float(x - fr[d]);
+}
+// FP32 separation x_a - x_b along d from partner b's record: the FP64
+// difference rounded once (2D), or the window constant of b's cell minus b's
+// frame offset (3D).
+template <int D>
+__device__ __forceinline__ float sep_x(const double* xa, const PartnerN<D>& q, int d) {
+  return float(xa[d] - q.x[d]);
+}
+__device__ __forceinline__ float sep_x(const double*, const PartnerN<3>& q, int d, const Win3& w) {
+  return w.k[d] - q.xo[d];
+}
+template <int D>
+__device__ __forceinline__ float sep_x(const double* xa, const PartnerH<D>& q, int d) {
+  return float(xa[d] - q.xh[d]);
+}
+__device__ __forceinline__ float sep_x(const double*, const PartnerH<3>& q, int d, const Win3& w) {
+  return w.k[d] - q.xo[d];
+}
+// Frame origins of particle a (3D; nothing to compute in 2D).
 template <int D, typename R>
+__device__ __forceinline__ void particle_frame(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int64_t a,
+                                               double fr[3]) {
+  if constexpr (D == 3) cell_frame3(C, G, B.skey[a], fr);
+  else fr[0] = fr[1] = fr[2] = 0.0;
+}
+// Separation constants of particle a at position xa (3D FP32-split).
+template <int D, typename R>
+__device__ __forceinline__ FrameK3 particle_k(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int64_t a,
+                                              const double* xa) {
+  if constexpr (D == 3) return frame_k3(C, G, B.skey[a], xa);
+  else return FrameK3{};
+}
+
+// Neighbour-list partners b of particle a in a fixed order (stencil rows in
+// order, ascending sorted index inside a row; in 3D each row visited cell by
+// cell, which is the same order).  nc >= 0: the nc stored entries of the
+// membership store, read one ahead so the next load overlaps the current
+// window's pair bodies.  nc < 0: the windows are computed here from the
+// membership positions XM.  Each lane runs its own sequence, so all lanes of a
+// wave execute the (long) pair body together.  Fr (3D only): the iterator also
+// keeps the current window's separation constants from the particle's FrameK3.
+template <int D, typename R, bool Fr = false>
 struct Partners {
+  static_assert(!Fr || cell_windows<D, R>(), "cell frames come with cell windows");
   const SphBufs<R>& B;
   const CellGeom& G;
   const double* const xm0;  // membership positions, held by value
@@ -408,9 +517,12 @@ struct Partners {
   int e = 1, k = -1;
   int32_t w0 = 0, re = 0, cb = 0;
   uint32_t c = 0;
+  FrameK3 kt;        // Fr: the particle's separation constants
+  float wk[3];       // Fr: constants of the current window
+  int j = 3;         // 3D scan (nc < 0): cell of row k, 0..2 for ox = -1..1
 
   __device__ __forceinline__ Partners(const SphBufs<R>& B_, int64_t a_, int nc_, const CellGeom& G_,
-                                      const double* const* XM_, double dist2_)
+                                      const double* const* XM_, double dist2_, const FrameK3* kt_ = nullptr)
       : B(B_),
         G(G_),
         xm0(XM_[0]),
@@ -421,6 +533,7 @@ struct Partners {
         nc(nc_),
         pe(B_.NE + a_),
         nxt(nc_ > 0 ? *pe : 0ull) {
+    if constexpr (Fr) kt = *kt_;
     fill();
   }
 
@@ -434,28 +547,76 @@ struct Partners {
     if (c == 0) fill();
     return b;
   }
+  // As take(), with the separation constants of the partner's window.
+  __device__ __forceinline__ int32_t take(Win3& w) {
+    w.k[0] = wk[0];
+    w.k[1] = wk[1];
+    w.k[2] = wk[2];
+    return take();
+  }
+  __device__ __forceinline__ int32_t take(NoWin&) { return take(); }
+
+  __device__ __forceinline__ void set_window(uint32_t code) {
+    if constexpr (Fr) {
+      // base-3 digits of code < 27: q / 3 = (q · 11) >> 5 for q < 27
+      const uint32_t q3 = (code * 11u) >> 5, q9 = (q3 * 11u) >> 5;
+      const uint32_t o1[3] = {code - 3u * q3, q3 - 3u * q9, q9};
+#pragma unroll
+      for (int d = 0; d < 3; ++d) wk[d] = fmaf(float(o1[d]), -kt.cf[d], kt.km[d]);
+    }
+  }
 
   // c == 0: loads the next window that has members (c stays 0 at the end).
   __device__ __forceinline__ void fill() {
     if (nc >= 0) {
       if (nxt == 0) return;  // stored entries always have member bits
-      c = uint32_t(nxt);
+      if constexpr (cell_windows<D, R>()) {
+        c = uint32_t(nxt) & kWinMask3;
+        set_window(uint32_t(nxt) >> kWin3);
+      } else {
+        c = uint32_t(nxt);
+      }
       cb = int32_t(nxt >> 32);
       pe += B.n;
       nxt = e < nc ? *pe : 0ull;
       ++e;
       return;
     }
-    while (c == 0) {
-      if (w0 >= re) {
-        if (++k >= Rows<D>::N) return;
-        row_range<D>(G, B.start, B.skey[a], k, w0, re);
-        continue;
+    if constexpr (cell_windows<D, R>()) {
+      // stencil cell by cell, windows of kWin3 candidates inside a cell
+      while (c == 0) {
+        if (w0 >= re) {
+          if (++j > 2) {
+            j = 0;
+            if (++k >= Rows<3>::N) return;
+          }
+          int64_t cx, cy, cz;
+          cell_xyz<3>(G, B.skey[a], cx, cy, cz);
+          const int64_t x = cx + j - 1, y = cy + (k % 3) - 1, z = cz + (k / 3) - 1;
+          const bool in = x >= 0 && x < G.grid[0] && y >= 0 && y < G.grid[1] && z >= 0 && z < G.grid[2];
+          const int64_t cell = x + G.grid[0] * (y + G.grid[1] * z);
+          w0 = in ? B.start[cell] : 0;
+          re = in ? B.start[cell + 1] : 0;
+          set_window(win_code3(j - 1, (k % 3) - 1, (k / 3) - 1));
+          continue;
+        }
+        const int32_t hi = re < w0 + kWin3 ? re : w0 + kWin3;
+        c = window_bits<3>(xm0, xm1, xm2, xm0[a], xm1[a], xm2[a], a, w0, hi, dist2);
+        cb = w0;
+        w0 = hi;
       }
-      const int32_t hi = re < w0 + 32 ? re : w0 + 32;
-      c = window_bits<D>(xm0, xm1, xm2, xm0[a], xm1[a], D == 3 ? xm2[a] : 0.0, a, w0, hi, dist2);
-      cb = w0;
-      w0 = hi;
+    } else {
+      while (c == 0) {
+        if (w0 >= re) {
+          if (++k >= Rows<D>::N) return;
+          row_range<D>(G, B.start, B.skey[a], k, w0, re);
+          continue;
+        }
+        const int32_t hi = re < w0 + 32 ? re : w0 + 32;
+        c = window_bits<D>(xm0, xm1, xm2, xm0[a], xm1[a], D == 3 ? xm2[a] : 0.0, a, w0, hi, dist2);
+        cb = w0;
+        w0 = hi;
+      }
     }
   }
 };
@@ -479,18 +640,28 @@ __device__ __forceinline__ void for_each_partner(const SphBufs<R>& B, int64_t a,
 // the body needs of partner b.  Pipe: the fetch of the next partner is issued
 // before the body of the current one runs, so the loads overlap the pair
 // arithmetic (at the cost of the registers holding both partners' data).
-template <bool Pipe, int D, typename R, typename Fetch, typename F>
+// Frames (3D FP32-split readers of partner records): kt holds the particle's
+// separation constants and the body receives the partner's window constants
+// as a third argument.
+template <bool Pipe, int D, typename R, bool Frames = false, typename Fetch, typename F>
 __device__ __forceinline__ void for_each_partner_fetch(const SphBufs<R>& B, int64_t a, int nc, const CellGeom& G,
                                                        const double* const* XM, double dist2, Fetch&& fetch,
-                                                       F&& body) {
-  Partners<D, R> it(B, a, nc, G, XM, dist2);
+                                                       F&& body, const FrameK3* kt = nullptr) {
+  constexpr bool Fr = Frames && D == 3 && std::is_same<R, float>::value;
+  using W = typename std::conditional<Fr, Win3, NoWin>::type;
+  Partners<D, R, Fr> it(B, a, nc, G, XM, dist2, kt);
+  auto run = [&](const int32_t b, const auto& f, const W& w) {
+    if constexpr (Fr) body(b, f, w);
+    else body(b, f);
+  };
   if constexpr (!Pipe) {
     for (;;) {
       const bool m = it.has();
       if (!__any(m)) break;
       if (m) {
-        const int32_t b = it.take();
-        body(b, fetch(b));
+        W w;
+        const int32_t b = it.take(w);
+        run(b, fetch(b), w);
       }
     }
     return;
@@ -500,18 +671,19 @@ __device__ __forceinline__ void for_each_partner_fetch(const SphBufs<R>& B, int6
   // is a valid address, and skips the body
   bool m0 = it.has();
   if (!__any(m0)) return;
-  int32_t b0 = m0 ? it.take() : int32_t(a);
+  W w0{}, w1{};
+  int32_t b0 = m0 ? it.take(w0) : int32_t(a);
   auto f0 = fetch(b0);
   for (;;) {
     const bool m1 = it.has();
-    const int32_t b1 = m1 ? it.take() : b0;
+    const int32_t b1 = m1 ? it.take(w1) : b0;
     auto f1 = fetch(b1);
-    if (m0) body(b0, f0);
+    if (m0) run(b0, f0, w0);
     if (!__any(m1)) break;
     m0 = it.has();
-    b0 = m0 ? it.take() : b1;
+    b0 = m0 ? it.take(w0) : b1;
     f0 = fetch(b0);
-    if (m1) body(b1, f1);
+    if (m1) run(b1, f1, w1);
     if (!__any(m0)) break;
   }
 }

@@ -100,11 +100,14 @@ constexpr int kNmaskWave = 400;
 constexpr int kNmaskPlane = 1280;
 
 // Neighbour-list membership of the last rebuild, stored per particle as
-// (mask, base) entries: one 32-candidate window [base, base+32) of one stencil
-// row with its member bits in the low word.  Only windows with members are
-// stored, in row order.  Entry e of particle a is NE[e * n + a]; NC[a] is the
-// entry count, or -1 when the particle needs more than necap entries (its
-// partners are then found by scanning the stencil).
+// (mask, base) entries: one window of candidates [base, base + len) with its
+// member bits in the low word.  32-candidate windows of one stencil row, or in
+// 3D FP32-split windows of up to kWin3 candidates inside one stencil cell with
+// the cell's offset coded above the member bits (win_code3), which is what
+// the cell frames of its partner records need.  Only windows with members are
+// stored, in stencil order.  Entry e of particle a is NE[e * n + a]; NC[a]
+// is the entry count, or -1 when the particle needs more than necap entries
+// (its partners are then found by scanning the stencil).
 //
 // k_nmask is the 3D store (the 2D one runs inside k_reorder_nmask2 below):
 // the block stages the candidates' positions plane by plane (three stencil
@@ -129,16 +132,34 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
   __shared__ double lx[kNmaskPlane + 4], ly[kNmaskPlane + 4], lz[kNmaskPlane + 4];  // +4: window reads run up to 3 past a row's end
   __shared__ RowUnion u;
   double xa0 = 0.0, xa1 = 0.0, xa2 = 0.0;
-  Rows<D> rows{};
+  int64_t ix = 0, iy = 0, iz = 0;
   if (active) {
     xa0 = XM[0][a];
     xa1 = XM[1][a];
     xa2 = XM[2][a];
-    rows = stencil_rows<D>(G, B.start, B.skey[a]);
+    cell_xyz<3>(G, B.skey[a], ix, iy, iz);
   }
+  const int64_t gx = G.grid[0], gy = G.grid[1], gz = G.grid[2], nce = G.ncells();
   int cnt = 0;
 #pragma unroll
   for (int pl = 0; pl < 3; ++pl) {  // rows 3·pl .. 3·pl+2 lie in the plane dz = pl - 1
+    // the cell boundaries of the particle's three rows in this plane, loaded
+    // together before the staging: cell ox = j - 1 of row kk is [sb[kk][j],
+    // sb[kk][j + 1]) when it lies in the grid (indices clamped for the load)
+    const int64_t z = iz + pl - 1;
+    int32_t sb[3][4];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+      const int64_t y = iy + kk - 1;
+      const int64_t row = gx * (y + gy * z);
+      const bool in = active && y >= 0 && y < gy && z >= 0 && z < gz;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        int64_t i = row + ix - 1 + t;
+        i = i < 0 ? 0 : (i > nce ? nce : i);
+        sb[kk][t] = in ? B.start[i] : 0;
+      }
+    }
     if (threadIdx.x == 0) row_union(u, B.start, B.skey, G, B.n, a0, pl - 1);
     __syncthreads();
     const int32_t total = u.uo[3];
@@ -156,18 +177,40 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
     if (active) {
 #pragma unroll
       for (int kk = 0; kk < 3; ++kk) {
-        const int k = 3 * pl + kk;
+        const int64_t y = iy + kk - 1;
+        if (y < 0 || y >= gy || z < 0 || z >= gz) continue;
         const int32_t off = u.uo[kk] - u.ub[kk];  // candidate b at l?[b + off]
-        const int32_t rb = rows.b[k], re = rows.e[k];
-        for (int32_t w0 = rb; w0 < re; w0 += 32) {
-          const int32_t hi = re < w0 + 32 ? re : w0 + 32;
-          const uint32_t mk =
-              staged ? window_bits_lds3(lx + (w0 + off), ly + (w0 + off), lz + (w0 + off), xa0, xa1, xa2, a, w0, hi,
-                                        dist2)
-                     : window_bits<D>(XM[0], XM[1], XM[2], xa0, xa1, xa2, a, w0, hi, dist2);
-          if (mk != 0) {
-            if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
-            ++cnt;
+        auto test = [&](int32_t w0, int32_t hi) {
+          return staged ? window_bits_lds3(lx + (w0 + off), ly + (w0 + off), lz + (w0 + off), xa0, xa1, xa2, a, w0, hi,
+                                           dist2)
+                        : window_bits<D>(XM[0], XM[1], XM[2], xa0, xa1, xa2, a, w0, hi, dist2);
+        };
+        if constexpr (cell_windows<D, R>()) {
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {  // the row's cells ox = j - 1, in sorted order
+            const int64_t x = ix + j - 1;
+            if (x < 0 || x >= gx) continue;
+            const int32_t cb = sb[kk][j], ce = sb[kk][j + 1];
+            const uint64_t code = uint64_t(win_code3(j - 1, kk - 1, pl - 1)) << kWin3;
+            for (int32_t w0 = cb; w0 < ce; w0 += kWin3) {
+              const int32_t hi = ce < w0 + kWin3 ? ce : w0 + kWin3;
+              const uint32_t mk = test(w0, hi);
+              if (mk != 0) {
+                if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | code | mk;
+                ++cnt;
+              }
+            }
+          }
+        } else {  // 32-candidate windows over the whole row
+          const int32_t rb = ix >= 1 ? sb[kk][0] : sb[kk][1];
+          const int32_t re = ix + 1 < gx ? sb[kk][3] : sb[kk][2];
+          for (int32_t w0 = rb; w0 < re; w0 += 32) {
+            const int32_t hi = re < w0 + 32 ? re : w0 + 32;
+            const uint32_t mk = test(w0, hi);
+            if (mk != 0) {
+              if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
+              ++cnt;
+            }
           }
         }
       }
@@ -204,8 +247,8 @@ __global__ void k_post_rebuild(SS ss) {
 // New sorted position s takes the state of old position perm[s]; returns the
 // particle's position (xs) for a membership test that follows.
 template <int D, typename R>
-__device__ __forceinline__ void reorder_one(const SphBufs<R>& B, const SphConst& C, const StepFlags& F,
-                                            const StepCtl* ctl, int64_t s, double* xs) {
+__device__ __forceinline__ void reorder_one(const SphBufs<R>& B, const SphConst& C, const CellGeom& G,
+                                            const StepFlags& F, const StepCtl* ctl, int64_t s, double* xs) {
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity, pq = pp ^ 1;
   const int32_t src = B.perm[s];
   R vs[3];
@@ -227,9 +270,11 @@ __device__ __forceinline__ void reorder_one(const SphBufs<R>& B, const SphConst&
   const int32_t pt = B.PT[pp][src];
   B.RHO[q][s] = rv;
   typename Records<D, R>::N rec{};
+  double fr[3];
+  particle_frame<D, R>(B, C, G, s, fr);  // s's cell in the new order
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    rec.x[k] = xs[k];
+    put_x(rec, k, xs[k], fr);
     rec.v[k] = vs[k];
   }
   rec.rho = rv;
@@ -254,7 +299,7 @@ __global__ void k_reorder(SS ss) {
   if (ctl->rebuild == 0) return;
   if (a >= B.n) return;
   double xs[3];
-  reorder_one<D, R>(B, C, F, ctl, a, xs);
+  reorder_one<D, R>(B, C, G, F, ctl, a, xs);
 }
 
 // window_bits for candidates read through the new order: candidate b (a
@@ -375,7 +420,7 @@ __global__ __launch_bounds__(256) void k_reorder_nmask2(SS ss) {
   if (a0 >= B.n) return;  // whole block
   __shared__ double2 lds[kBlock / 64][kNmaskWave + 4];  // +4: window reads run up to 3 past a row's end
   double xa[3] = {0.0, 0.0, 0.0};
-  if (a < B.n) reorder_one<2, R>(B, C, F, ctl, a, xa);
+  if (a < B.n) reorder_one<2, R>(B, C, G, F, ctl, a, xa);
   nmask2_perm<R>(B, C, G, ctl, a, a0, xa, lds);
 }
 
@@ -455,12 +500,14 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
     const float rhoa = C.f_rho0 + RHO[a];
     const float fdt = float(dt);
     Sum<float> sum[3];
-    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+    const FrameK3 kt = particle_k<D, R>(B, C, G, a, xa);
+    for_each_partner_fetch<true, D, R, true>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n,
+                                                                         const auto&... win) {
       if (n.pt <= 0) return;
       float dx[3], dv[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        dx[d] = float(xa[d] - n.x[d]);
+        dx[d] = sep_x(xa, n, d, win...);
         dv[d] = va[d] - n.v[d];
       }
       const float rhob = C.f_rho0 + n.rho;
@@ -484,7 +531,7 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
 #pragma unroll
         for (int d = 0; d < D; ++d) sum[d].add(tvar * dx[d]);
       }
-    });
+    }, &kt);
 #pragma unroll
     for (int d = 0; d < D; ++d) B.DV[d][a] = sum[d].get();
   }
@@ -500,6 +547,8 @@ __global__ void k_dpc_apply(SS ss) {
   const int q = ctl->parity ^ 1, pp = ctl->pparity;
   const double dt = ctl->dt;
   if (a >= B.n || B.PT[pp][a] <= 1) return;
+  double fr[3];
+  particle_frame<D, R>(B, C, G, a, fr);
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     R dv = B.DV[d][a];
@@ -508,7 +557,7 @@ __global__ void k_dpc_apply(SS ss) {
     B.V[q][d][a] = nv;
     B.X[q][d][a] = nx;
     rec_n<D, R>(B.PRN)[a].v[d] = nv;
-    rec_n<D, R>(B.PRN)[a].x[d] = nx;
+    put_x(rec_n<D, R>(B.PRN)[a], d, nx, fr);
   }
 }
 
@@ -523,9 +572,11 @@ __global__ void k_xsph_rec(SS ss) {
   const int p = ctl->parity, q = p ^ 1, pp = ctl->pparity;
   if (a >= B.n) return;
   typename Records<D, R>::N rec{};
+  double fr[3];
+  particle_frame<D, R>(B, C, G, a, fr);
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    rec.x[d] = B.X[p][d][a];
+    put_x(rec, d, B.X[p][d][a], fr);
     rec.v[d] = B.V[q][d][a];
   }
   rec.rho = B.RHO[q][a];
@@ -583,25 +634,29 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
 #pragma clang fp contract(fast)
     const float rhoa = C.f_rho0 + RHO[a];
     Sum<float> sum[3];
-    for_each_partner_fetch<true, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n) {
+    const FrameK3 kt = particle_k<D, R>(B, C, G, a, xa);
+    for_each_partner_fetch<true, D, R, true>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n,
+                                                                         const auto&... win) {
       if (n.pt <= 1) return;
       float dx[3];
 #pragma unroll
-      for (int d = 0; d < D; ++d) dx[d] = float(xa[d] - n.x[d]);
+      for (int d = 0; d < D; ++d) dx[d] = sep_x(xa, n, d, win...);
       const float r = __builtin_amdgcn_sqrtf(dotf<D>(dx, dx));
       const float w = W_f(r * C.f_Hinv, C.f_wnorm);
       const float xs = C.f_xsph_c * w * __builtin_amdgcn_rcpf(rhoa + (C.f_rho0 + n.rho));
 #pragma unroll
       for (int d = 0; d < D; ++d) sum[d].add((n.v[d] - va[d]) * xs);
-    });
+    }, &kt);
 #pragma unroll
     for (int d = 0; d < D; ++d) out[d] = double(sum[d].get());
   }
+  double fr[3];
+  particle_frame<D, R>(B, C, G, a, fr);
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const double nx = B.X[q][d][a] + out[d] * dt;
     B.X[q][d][a] = nx;
-    rec_n<D, R>(B.PRN)[a].x[d] = nx;
+    put_x(rec_n<D, R>(B.PRN)[a], d, nx, fr);
   }
 }
 

@@ -179,14 +179,15 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
     const float Pa = B.PN[a];
     using Nb = PartnerN<D>;
     auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b); };
+    const FrameK3 kt = particle_k<D, R>(B, C, G, a, xa);
     // ∇W = dx·dwr, so every term along ∇W is gathered into one coefficient fg
     // of dx, and dx·∇W = r²·dwr, dv·∇W = (dv·dx)·dwr.
-    for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
+    for_each_partner_fetch<Pipe, D, R, true>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q, const auto&... win) {
       const int32_t ptb = q.pt;
       float dx[3], dv[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        dx[d] = float(xa[d] - q.x[d]);
+        dx[d] = sep_x(xa, q, d, win...);
         dv[d] = va[d] - q.v[d];
       }
       const float r2 = dotf<D>(dx, dx);
@@ -246,7 +247,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       }
 #pragma unroll
       for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
-    });
+    }, &kt);
     float av[3];
 #pragma unroll
     for (int d = 0; d < D; ++d) av[d] = acc[d].get();
@@ -275,6 +276,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
     B.RH[a] = rv;
     B.PH[a] = eos_f(rv, pta, C);
     PartnerHX<D> rec{};
+    double fr[3];
+    particle_frame<D, R>(B, C, G, a, fr);
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       float vv = vh0[d];
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       const double xh = xh0[d] + double(vv) * (dt * 0.5);
       B.VH[d][a] = vv;
       B.XH[d][a] = xh;
-      rec.h.xh[d] = xh;
+      put_x(rec.h, d, xh, fr);
       rec.h.vh[d] = vv;
       rec.dn[d] = float(xh - xa[d]);
     }
@@ -471,14 +474,15 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       float dna[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) dna[d] = float(xha[d] - xa[d]);
-      for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& r) {
+      const FrameK3 kt = particle_k<D, R>(B, C, G, a, xha);
+      for_each_partner_fetch<Pipe, D, R, true>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& r,
+                                                                            const auto&... win) {
         const PartnerH<D>& q = r.h;
         const int32_t ptb = q.pt;
         float dxn[3], dx[3], dv[3];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          const double dh = xha[d] - q.xh[d];
-          dx[d] = float(dh);
+          dx[d] = sep_x(xha, q, d, win...);
           dv[d] = vha[d] - q.vh[d];
           dxn[d] = derive ? dx[d] - dv[d] * fhdt : dx[d] - (dna[d] - r.dn[d]);
         }
@@ -548,7 +552,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
         }
 #pragma unroll
         for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
-      });
+      }, &kt);
 #pragma unroll
       for (int d = 0; d < D; ++d) av[d] = acc[d].get();
       av[1] = av[1] - C.f_g;
@@ -565,6 +569,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       const float ml = pta > 1 ? 1.0f : 0.0f;
       double xn[3], vn[3];
       PartnerN<D> rec{};
+      double fr[3];
+      particle_frame<D, R>(B, C, G, a, fr);
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         float vv = B.V[p][d][a];
@@ -581,7 +587,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
         vn[d] = nv;
         B.V[q][d][a] = nv;
         B.X[q][d][a] = xv;
-        rec.x[d] = xv;
+        put_x(rec, d, xv, fr);
         rec.v[d] = nv;
         if (F.corr_on) {
           B.VH2[d][a] = nv;
