@@ -35,11 +35,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 # cycles per SIMD, at the 2.4 GHz peak engine clock (1 wave64 instruction per
 # CU per cycle; the FP32 vector peak without packed math, 78.6 TFLOP/s in FMA)
 VALU_PEAK_GIPS = 256 * 2.4
-# The first call of more than 6 steps on a problem or batch times the pass
-# variants on its first 6 steps (4 code-generation variants, then 2 more block
-# sizes); a warmup call of at least 7 steps keeps that tuning out of the timed
-# call.
-TUNE_STEPS = 7
+# The first call of more than 15 steps on a problem or batch times the pass
+# variants on 15 steps (4 code-generation variants at 3 block sizes, the
+# occupancy cap, the two fastest again) after up to 16 untimed ones (a fresh
+# device warms up over its first dozen steps); a warmup call of 32 steps
+# keeps that tuning, fully warmed, out of the timed call.
+TUNE_STEPS = 32
 
 
 def log(msg: str) -> None:

@@ -799,38 +799,72 @@ void run_steps(sph_s* const* P, unsigned count, LaunchSet& L, hipStream_t st, in
   };
   int64_t done = 0;
   const char* no_tune = std::getenv("GCLSPH_AUTOTUNE");
-  if (!forced && L.pass_variant[0] < 0 && !timer && nsteps > kTuneSteps && !(no_tune && no_tune[0] == '0')) {
-    // Autotune on the first kTuneSteps steps of the call (live steps: every
-    // variant computes the same values), timing both passes with events: the
-    // kPassCodegen code-generation variants at 256 threads per block first,
-    // then the fastest of them per pass at 128 and 512 threads.  The fastest
-    // variant of each pass is kept for this launch set.
-    std::vector<hipEvent_t> ev(3 * kTuneSteps, nullptr);
+  if (!forced && L.pass_variant[0] < 0 && !timer && nsteps > kTuneTimed && !(no_tune && no_tune[0] == '0')) {
+    // Autotune on the first steps of the call (live steps: every variant
+    // computes the same values).  Up to kTuneWarm untimed steps first, as
+    // many as the call has beyond the kTuneTimed timed ones: the first dozen
+    // steps of a run on a fresh device run up to 10 % slower while it warms
+    // up, which would favour the variants timed late.  Then both passes are
+    // timed with events: the kPassCodegen code-generation variants at each
+    // block size, then the fastest of them per pass with the occupancy cap
+    // and the two fastest once more (timings of variants within a few per
+    // cent of each other are noisy; a variant's time is its shortest).  The
+    // steps of each stage are queued back to back, as in production, and
+    // read after the stage.  The variant of each pass with the shortest time
+    // is kept for this launch set.
+    const int warm = int(std::min<int64_t>(kTuneWarm, nsteps - kTuneTimed));
+    std::vector<hipEvent_t> ev(3 * (kTuneWarm + kTuneTimed), nullptr);
     for (auto& e : ev) GCL_HIP(hipEventCreate(&e));
-    float best[2] = {0, 0};
-    auto time_step = [&](int k, int v1, int v2) {
+    std::vector<float> tv[2] = {std::vector<float>(kPassVariants, -1.0f), std::vector<float>(kPassVariants, -1.0f)};
+    std::vector<std::pair<int, int>> queued;  // (step, variant pass 1 * kPassVariants + variant pass 2)
+    auto time_step = [&](int& k, int v1, int v2) {
+      queued.emplace_back(k, v1 * kPassVariants + v2);
       launch_step(cspm_at(k + 1), v1, v2, &ev[3 * size_t(k)]);
-      GCL_HIP(hipStreamSynchronize(st));
-      float t1 = 0, t2 = 0;
-      GCL_HIP(hipEventElapsedTime(&t1, ev[3 * size_t(k)], ev[3 * size_t(k) + 1]));
-      GCL_HIP(hipEventElapsedTime(&t2, ev[3 * size_t(k) + 1], ev[3 * size_t(k) + 2]));
-      if (k == 0 || t1 < best[0]) {
-        best[0] = t1;
-        pv[0] = v1;
-      }
-      if (k == 0 || t2 < best[1]) {
-        best[1] = t2;
-        pv[1] = v2;
-      }
+      ++k;
     };
-    for (int v = 0; v < kPassCodegen; ++v) time_step(v, v, v);
-    const int cg1 = pv[0], cg2 = pv[1];
-    for (int b = 1; b < kPassBlockSizes; ++b)
-      time_step(kPassCodegen + b - 1, cg1 + kPassCodegen * b, cg2 + kPassCodegen * b);
+    auto read_stage = [&] {
+      GCL_HIP(hipStreamSynchronize(st));
+      for (const auto& q : queued) {
+        const size_t k = size_t(q.first);
+        float t[2] = {0, 0};
+        GCL_HIP(hipEventElapsedTime(&t[0], ev[3 * k], ev[3 * k + 1]));
+        GCL_HIP(hipEventElapsedTime(&t[1], ev[3 * k + 1], ev[3 * k + 2]));
+        const int v[2] = {q.second / kPassVariants, q.second % kPassVariants};
+        for (int p = 0; p < 2; ++p)
+          if (tv[p][v[p]] < 0 || t[p] < tv[p][v[p]]) tv[p][v[p]] = t[p];
+      }
+      queued.clear();
+    };
+    // the fastest (rank 0) or second fastest (rank 1) timed variant of pass p
+    auto ranked = [&](int p, int rank) {
+      int b0 = -1, b1 = -1;
+      for (int v = 0; v < kPassVariants; ++v) {
+        if (tv[p][v] < 0) continue;
+        if (b0 < 0 || tv[p][v] < tv[p][b0]) {
+          b1 = b0;
+          b0 = v;
+        } else if (b1 < 0 || tv[p][v] < tv[p][b1]) {
+          b1 = v;
+        }
+      }
+      return rank == 0 || b1 < 0 ? b0 : b1;
+    };
+    const int nocap = kPassCodegen * kPassBlockSizes;
+    int k = 0;
+    for (; k < warm; ++k) launch_step(cspm_at(k + 1), pv[0], pv[1], nullptr);
+    for (int v = 0; v < nocap; ++v) time_step(k, v, v);
+    read_stage();
+    const int r0[2] = {ranked(0, 0), ranked(1, 0)}, r1[2] = {ranked(0, 1), ranked(1, 1)};
+    time_step(k, r0[0] % nocap + nocap, r0[1] % nocap + nocap);
+    time_step(k, r0[0], r0[1]);
+    time_step(k, r1[0], r1[1]);
+    read_stage();
     for (auto& e : ev) (void)hipEventDestroy(e);
+    pv[0] = ranked(0, 0);
+    pv[1] = ranked(1, 0);
     L.pass_variant[0] = pv[0];
     L.pass_variant[1] = pv[1];
-    done = kTuneSteps;
+    done = k;
   }
   const char* no_graph = std::getenv("GCLSPH_NO_GRAPH");
   const bool graph = !timer && nsteps - done >= 2 && !(no_graph && no_graph[0] == '1');

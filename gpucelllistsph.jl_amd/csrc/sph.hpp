@@ -272,16 +272,27 @@ struct LaunchSet {
 template <int D, typename R>
 void launch_rebuild(const LaunchSet& L, hipStream_t s);
 // Pair passes in one of kPassVariants variants that differ only in code
-// generation and launch shape (same results): variant = cg + 4·bs, with cg in
-// [0, 4) the prefetch / occupancy choice and bs in [0, 3) the block size
-// (256, 128, 512 threads).
+// generation and launch shape (same results): variant = cg + 4·bs + 12·oc,
+// with cg in [0, 4) the prefetch / occupancy-target choice, bs in [0, 3) the
+// block size (256, 128, 512 threads) and oc in [0, 2) an occupancy cap (1: at
+// most kPassCapWaves waves per SIMD, by launching with unused LDS).
 constexpr int kPassCodegen = 4;
 constexpr int kPassBlockSizes = 3;
-constexpr int kPassVariants = kPassCodegen * kPassBlockSizes;
-// Steps the autotuner times: every code-generation variant at 256 threads,
-// then the best of them at each other block size.
-constexpr int kTuneSteps = kPassCodegen + kPassBlockSizes - 1;
-constexpr int pass_block_size(int variant) { return variant / kPassCodegen == 1 ? 128 : variant / kPassCodegen == 2 ? 512 : 256; }
+constexpr int kPassCaps = 2;
+constexpr int kPassCapWaves = 4;
+constexpr int kPassVariants = kPassCodegen * kPassBlockSizes * kPassCaps;
+// Steps the autotuner times: every code-generation variant at every block
+// size (the best shape at one block size need not be the best at another),
+// then the best of them with the cap and the two best once more; before them
+// up to kTuneWarm untimed steps (as many as the call has to spare).
+constexpr int kTuneTimed = kPassCodegen * kPassBlockSizes + kPassCaps - 1 + 2;
+constexpr int kTuneWarm = 16;
+constexpr int pass_block_size(int variant) {
+  return (variant / kPassCodegen) % kPassBlockSizes == 1 ? 128
+         : (variant / kPassCodegen) % kPassBlockSizes == 2 ? 512
+                                                           : 256;
+}
+constexpr int pass_cap(int variant) { return variant / (kPassCodegen * kPassBlockSizes); }
 template <int D, typename R>
 void launch_pass1(const LaunchSet& L, hipStream_t s, int variant);
 template <int D, typename R>

@@ -333,6 +333,31 @@ __device__ __forceinline__ uint32_t window_bits(const double* x0, const double* 
   return mk;
 }
 
+// window_bits for the stencil scan inside the pair loops (a particle whose
+// windows overflowed the membership store): one candidate at a time, so the
+// scan adds few registers to the loop around it (the four-wide form above
+// sets the register count, and so the occupancy, of every pair kernel).  Same
+// expression, same bits.
+template <int D>
+__device__ __forceinline__ uint32_t window_bits_lean(const double* x0, const double* x1, const double* x2,
+                                                     double xa0, double xa1, double xa2, int64_t a, int32_t lo,
+                                                     int32_t hi, double dist2) {
+#pragma clang fp contract(off)
+  uint32_t mk = 0;
+#pragma clang loop unroll(disable)
+  for (int32_t b = lo; b < hi; ++b) {
+    const double d0 = xa0 - x0[b];
+    const double d1 = xa1 - x1[b];
+    double r2 = d0 * d0 + d1 * d1;
+    if (D == 3) {
+      const double d2 = xa2 - x2[b];
+      r2 = r2 + d2 * d2;
+    }
+    mk |= uint32_t(r2 < dist2 && b != a) << (b - lo);
+  }
+  return mk;
+}
+
 // Sorted range [b, e) of stencil row k (dy = k%3 - 1, dz = k/3 - 1) of cell c;
 // empty when the row lies outside the grid.
 template <int D>
@@ -601,7 +626,7 @@ struct Partners {
           continue;
         }
         const int32_t hi = re < w0 + kWin3 ? re : w0 + kWin3;
-        c = window_bits<3>(xm0, xm1, xm2, xm0[a], xm1[a], xm2[a], a, w0, hi, dist2);
+        c = window_bits_lean<3>(xm0, xm1, xm2, xm0[a], xm1[a], xm2[a], a, w0, hi, dist2);
         cb = w0;
         w0 = hi;
       }
@@ -613,7 +638,7 @@ struct Partners {
           continue;
         }
         const int32_t hi = re < w0 + 32 ? re : w0 + 32;
-        c = window_bits<D>(xm0, xm1, xm2, xm0[a], xm1[a], D == 3 ? xm2[a] : 0.0, a, w0, hi, dist2);
+        c = window_bits_lean<D>(xm0, xm1, xm2, xm0[a], xm1[a], D == 3 ? xm2[a] : 0.0, a, w0, hi, dist2);
         cb = w0;
         w0 = hi;
       }

@@ -21,6 +21,8 @@
 // (r < r0, cond < 0, clamps) see the same values as the reference would.
 // R = float: the FP32-split path (density stored as ρ-ρ0, FP32 pair terms
 // summed in FP64, Float64 positions and Δt).
+#include <atomic>
+
 #include "sph_device.hpp"
 
 namespace gclsph {
@@ -626,52 +628,76 @@ unsigned pass_grid(const LaunchSet& L) {
   return BS == 256 ? L.pblocks : BS == 128 ? 2 * L.pblocks : L.pblocks512;
 }
 template <int D, typename R, bool Pipe, int W, int BS>
-void pass1_as(const LaunchSet& L, hipStream_t s) {
-  with_scenes<R>(L, [&](auto ss) { k_pass1<D, R, decltype(ss), Pipe, W, BS><<<pass_grid<BS>(L), BS, 0, s>>>(ss); });
+void pass1_as(const LaunchSet& L, hipStream_t s, size_t lds) {
+  with_scenes<R>(L, [&](auto ss) { k_pass1<D, R, decltype(ss), Pipe, W, BS><<<pass_grid<BS>(L), BS, lds, s>>>(ss); });
 }
 template <int D, typename R, bool Pipe, int W, int BS>
-void pass2_as(const LaunchSet& L, hipStream_t s) {
-  with_scenes<R>(L, [&](auto ss) { k_pass2<D, R, decltype(ss), Pipe, W, BS><<<pass_grid<BS>(L), BS, 0, s>>>(ss); });
+void pass2_as(const LaunchSet& L, hipStream_t s, size_t lds) {
+  with_scenes<R>(L, [&](auto ss) { k_pass2<D, R, decltype(ss), Pipe, W, BS><<<pass_grid<BS>(L), BS, lds, s>>>(ss); });
+}
+// Unused dynamic LDS per block of BS threads that leaves room for at most
+// kPassCapWaves waves per SIMD (B = 4·kPassCapWaves·64/BS blocks per CU: B
+// blocks fit the CU's LDS, B + 1 do not); 0 when the cap would need more LDS
+// than a block may have.  Queried once per device.
+size_t cap_lds(int bs) {
+  static std::atomic<int64_t> per_cu[64], per_block[64];  // 0: not queried yet
+  int dev = 0;
+  GCL_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return 0;
+  if (per_cu[dev].load() == 0) {
+    int cu = 0, blk = 0;
+    GCL_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    GCL_HIP(hipDeviceGetAttribute(&blk, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    per_block[dev].store(blk);
+    per_cu[dev].store(cu);
+  }
+  const size_t cu = size_t(per_cu[dev].load()), blk = size_t(per_block[dev].load());
+  const size_t blocks = size_t(4 * kPassCapWaves * 64 / bs);
+  const size_t lds = cu / (blocks + 1) + 1024;  // the static LDS of a pass block is below 1 KiB
+  return blocks * lds <= cu && lds <= blk ? lds : 0;
 }
 // code generation cg: 0 pipelined; 1 not pipelined; 2 pipelined, >= 3
 // waves/SIMD; 3 not pipelined, >= 5 waves/SIMD
 template <int D, typename R, int BS>
-void pass1_cg(const LaunchSet& L, hipStream_t s, int cg) {
+void pass1_cg(const LaunchSet& L, hipStream_t s, int cg, size_t lds) {
   switch (cg) {
-    case 1: pass1_as<D, R, false, 1, BS>(L, s); break;
-    case 2: pass1_as<D, R, true, 3, BS>(L, s); break;
-    case 3: pass1_as<D, R, false, 5, BS>(L, s); break;
-    default: pass1_as<D, R, true, 1, BS>(L, s); break;
+    case 1: pass1_as<D, R, false, 1, BS>(L, s, lds); break;
+    case 2: pass1_as<D, R, true, 3, BS>(L, s, lds); break;
+    case 3: pass1_as<D, R, false, 5, BS>(L, s, lds); break;
+    default: pass1_as<D, R, true, 1, BS>(L, s, lds); break;
   }
 }
 template <int D, typename R, int BS>
-void pass2_cg(const LaunchSet& L, hipStream_t s, int cg) {
+void pass2_cg(const LaunchSet& L, hipStream_t s, int cg, size_t lds) {
   switch (cg) {
-    case 1: pass2_as<D, R, false, 1, BS>(L, s); break;
-    case 2: pass2_as<D, R, true, 3, BS>(L, s); break;
-    case 3: pass2_as<D, R, false, 5, BS>(L, s); break;
-    default: pass2_as<D, R, true, 1, BS>(L, s); break;
+    case 1: pass2_as<D, R, false, 1, BS>(L, s, lds); break;
+    case 2: pass2_as<D, R, true, 3, BS>(L, s, lds); break;
+    case 3: pass2_as<D, R, false, 5, BS>(L, s, lds); break;
+    default: pass2_as<D, R, true, 1, BS>(L, s, lds); break;
   }
 }
 }  // namespace
 
-// variant = cg + kPassCodegen·bs (bs 0: 256, 1: 128, 2: 512 threads per block)
+// variant = cg + kPassCodegen·bs + kPassCodegen·kPassBlockSizes·oc (bs 0: 256,
+// 1: 128, 2: 512 threads per block; oc 1: occupancy cap)
 template <int D, typename R>
 void launch_pass1(const LaunchSet& L, hipStream_t s, int variant) {
-  const int cg = variant % kPassCodegen;
-  switch (pass_block_size(variant)) {
-    case 128: pass1_cg<D, R, 128>(L, s, cg); break;
-    case 512: pass1_cg<D, R, 512>(L, s, cg); break;
-    default: pass1_cg<D, R, 256>(L, s, cg); break;
+  const int cg = variant % kPassCodegen, bs = pass_block_size(variant);
+  const size_t lds = pass_cap(variant) ? cap_lds(bs) : 0;
+  switch (bs) {
+    case 128: pass1_cg<D, R, 128>(L, s, cg, lds); break;
+    case 512: pass1_cg<D, R, 512>(L, s, cg, lds); break;
+    default: pass1_cg<D, R, 256>(L, s, cg, lds); break;
   }
 }
 template <int D, typename R>
 void launch_pass2(const LaunchSet& L, hipStream_t s, int variant) {
-  const int cg = variant % kPassCodegen;
-  switch (pass_block_size(variant)) {
-    case 128: pass2_cg<D, R, 128>(L, s, cg); break;
-    case 512: pass2_cg<D, R, 512>(L, s, cg); break;
-    default: pass2_cg<D, R, 256>(L, s, cg); break;
+  const int cg = variant % kPassCodegen, bs = pass_block_size(variant);
+  const size_t lds = pass_cap(variant) ? cap_lds(bs) : 0;
+  switch (bs) {
+    case 128: pass2_cg<D, R, 128>(L, s, cg, lds); break;
+    case 512: pass2_cg<D, R, 512>(L, s, cg, lds); break;
+    default: pass2_cg<D, R, 256>(L, s, cg, lds); break;
   }
 }
 

@@ -329,8 +329,8 @@ class SPHProblem:
 
     def pass_variants(self) -> dict:
         """The pair-pass variants the autotuner chose for this problem's own
-        calls (variant = codegen + 4·blocksize index) and their block sizes;
-        -1 until tuned."""
+        calls (variant = codegen + 4·blocksize index + 12·occupancy cap) and
+        their block sizes; -1 until tuned."""
         out = (C.c_int32 * 4)()
         check(lib().sph_get_pass_variants(self._h, out))
         return {"pass1": int(out[0]), "pass2": int(out[1]), "pass1_threads": int(out[2]),

@@ -192,11 +192,13 @@ int sph_get_timing(sph_t h, double* ms, int64_t* launches, int cap, int* nstages
 
 /* The pair-pass variants the autotuner chose for a problem (its own calls) or
  * a batch (no counterpart in the reference: the autotuner is this library's).
- * The first call of more than 6 steps times them on its first 6 steps.
+ * The first call of more than 15 steps times them on 15 of its first 31
+ * steps, after up to 16 untimed ones (the device warming up).
  * out4: variant of pass 1, variant of pass 2, threads per block of pass 1,
- * of pass 2; -1 until tuned.  A variant is codegen + 4·b, codegen in [0, 4)
- * (partner prefetch on/off, occupancy target) and b the block size
- * (0: 256, 1: 128, 2: 512 threads); every variant gives identical results. */
+ * of pass 2; -1 until tuned.  A variant is codegen + 4·b + 12·c, codegen in
+ * [0, 4) (partner prefetch on/off, occupancy target), b the block size (0:
+ * 256, 1: 128, 2: 512 threads) and c an occupancy cap (1: at most 4 waves per
+ * SIMD); every variant gives identical results. */
 int sph_get_pass_variants(sph_t h, int32_t* out4);
 int sph_batch_get_pass_variants(sph_batch_t b, int32_t* out4);
 

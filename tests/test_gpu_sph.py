@@ -409,9 +409,9 @@ def test_config2_3d_f32split_vs_f64_500_steps(G, S):
 @pytest.mark.parametrize("prec", ["f64", "f32split"])
 def test_pass_variants_bit_identical(G, monkeypatch, prec):
     """The pair passes exist in variants (partner prefetch, occupancy target,
-    and 128, 256 or 512 threads per block: variant = codegen + 4·blocksize)
-    that the step driver times on live steps and picks from; every variant,
-    forced or autotuned, must give bit-identical steps."""
+    128, 256 or 512 threads per block, and an occupancy cap: variant = codegen
+    + 4·blocksize + 12·cap) that the step driver times on live steps and picks
+    from; every variant, forced or autotuned, must give bit-identical steps."""
     sc, steps, ts = golden_cases()["dam2d_allphys"]
 
     def run(variant):
@@ -421,16 +421,16 @@ def test_pass_variants_bit_identical(G, monkeypatch, prec):
             monkeypatch.setenv("GCLSPH_PASS_VARIANT", variant)
         cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
         prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, v=sc.v, precision=prec)
-        out = [G.stepsolve_(prob, 9, timestepping=ts), G.stepsolve_(prob, 3, timestepping=ts)]
+        out = [G.stepsolve_(prob, 30, timestepping=ts), G.stepsolve_(prob, 3, timestepping=ts)]
         if variant is None:  # the tuner ran on the first call and recorded its choice
             pv = prob.pass_variants()
-            assert 0 <= pv["pass1"] < 12 and 0 <= pv["pass2"] < 12
+            assert 0 <= pv["pass1"] < 24 and 0 <= pv["pass2"] < 24
             assert pv["pass1_threads"] in (128, 256, 512) and pv["pass2_threads"] in (128, 256, 512)
         return out + [np.asarray(gpu_fields(G, prob)[f]) for f in FIELDS]
 
     ref = run("0,0")
     for v in ("1,1", "2,2", "3,3", "1,3", "4,4", "5,5", "6,6", "7,7", "8,8", "9,9", "10,10", "11,11", "4,9", "8,1",
-              None):
+              "12,12", "17,19", "22,23", "14,0", None):
         got = run(v)
         for a, b in zip(got, ref):
             assert np.array_equal(np.asarray(a), np.asarray(b)), v
