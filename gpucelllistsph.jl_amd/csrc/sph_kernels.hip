@@ -244,8 +244,11 @@ __global__ void k_post_rebuild(SS ss) {
   }
 }
 
-// New sorted position s takes the state of old position perm[s]; returns the
-// particle's position (xs) for a membership test that follows.
+// New sorted position s takes the state of old position perm[s] (its
+// position is already there: the rank kernel moves the positions, and moving
+// the other fields there too measured slower: its stores are scattered where
+// the loads here are); returns the particle's position (xs) for a membership
+// test that follows.
 template <int D, typename R>
 __device__ __forceinline__ void reorder_one(const SphBufs<R>& B, const SphConst& C, const CellGeom& G,
                                             const StepFlags& F, const StepCtl* ctl, int64_t s, double* xs) {
@@ -254,11 +257,10 @@ __device__ __forceinline__ void reorder_one(const SphBufs<R>& B, const SphConst&
   R vs[3];
 #pragma unroll
   for (int k = 0; k < D; ++k) {
-    const double xv = B.X[p][k][src];
+    const double xv = B.X[q][k][s];
     const R vv = B.V[p][k][src];
     xs[k] = xv;
     vs[k] = vv;
-    B.X[q][k][s] = xv;
     B.V[q][k][s] = vv;
     if (F.write_xrb) B.XRB[k][s] = xv;
     if (!F.synced_prev) {  // previous half-step state moves with its particle
@@ -302,45 +304,18 @@ __global__ void k_reorder(SS ss) {
   reorder_one<D, R>(B, C, G, F, ctl, a, xs);
 }
 
-// window_bits for candidates read through the new order: candidate b (a
-// sorted index) sits at old position perm[b] of x0/x1.  Reads past n (the
-// window reads run up to 3 past a row's end) are clamped to n - 1; their bits
-// are dropped.
-__device__ __forceinline__ uint32_t window_bits_perm(const double* x0, const double* x1, const int32_t* perm,
-                                                     int64_t n, double xa0, double xa1, int64_t a, int32_t lo,
-                                                     int32_t hi, double dist2) {
-#pragma clang fp contract(off)
-  const int32_t len = hi - lo;
-  const int32_t last = int32_t(n - 1);
-  uint32_t acc = 0;
-  for (int32_t j = 0; j < len; j += 4) {
-    double r[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int32_t b = lo + j + u < last ? lo + j + u : last;
-      const int32_t src = perm[b];
-      const double d0 = xa0 - x0[src];
-      const double d1 = xa1 - x1[src];
-      r[u] = d0 * d0 + d1 * d1;
-    }
-    acc = push_lt4(acc, r[0], r[1], r[2], r[3], dist2);
-  }
-  return window_mask(acc, len, a, lo);
-}
-
-// Membership store (k_nmask's 2D path) of particle a, reading the rebuild
-// positions through the new order from the positions before the reorder
-// (x_old[perm[b]] is the reordered x[b]), so it needs no other block's reorder
-// writes.  xa: a's rebuild position.  Wave-level only (no block barrier).
+// Membership store (k_nmask's 2D path) of particle a, from the rebuild
+// positions in the new order, which the rank kernel wrote (the other fields
+// move in this kernel, which needs no other block's writes).  xa: a's
+// rebuild position.  Wave-level only (no block barrier).
 template <typename R>
-__device__ __forceinline__ void nmask2_perm(const SphBufs<R>& B, const SphConst& C, const CellGeom& G,
-                                            const StepCtl* ctl, int64_t a, int64_t a0, const double* xa,
-                                            double2 (*lds)[kNmaskWave + 4]) {
+__device__ __forceinline__ void nmask2_new(const SphBufs<R>& B, const SphConst& C, const CellGeom& G,
+                                           const StepCtl* ctl, int64_t a, int64_t a0, const double* xa,
+                                           double2 (*lds)[kNmaskWave + 4]) {
   const bool active = a < B.n;
   const double dist2 = C.dist2;
-  const int p = ctl->parity;  // positions before the reorder
-  const double* XO[2] = {B.X[p][0], B.X[p][1]};
-  const int32_t* perm = B.perm;
+  const int q = ctl->parity ^ 1;  // the new order (committed after this kernel)
+  const double* XN[2] = {B.X[q][0], B.X[q][1]};
   const int cap = B.necap;
   uint64_t* out = B.NE + a;
   const int w = __builtin_amdgcn_readfirstlane(int(threadIdx.x >> 6));
@@ -354,22 +329,19 @@ __device__ __forceinline__ void nmask2_perm(const SphBufs<R>& B, const SphConst&
   if (total <= kNmaskWave) {  // uniform over the wave
     double2* wl = lds[w];
     const int32_t d0 = u.ub[0] - u.uo[0], d1 = u.ub[1] - u.uo[1], d2 = u.ub[2] - u.uo[2];
-    // four slots per lane per trip, so four dependent load chains (perm, then
-    // the positions) are in flight together; a slot past the union repeats
-    // the last valid one's loads and is not written
+    // four slots per lane per trip, their loads in flight together; a slot
+    // past the union repeats the last valid one's loads and is not written
     for (int32_t i0 = lane; i0 < total; i0 += 4 * 64) {
-      int32_t src[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int32_t i = i0 + 64 * q < total ? i0 + 64 * q : total - 1;
-        src[q] = perm[i + (i >= u.uo[2] ? d2 : (i >= u.uo[1] ? d1 : d0))];
-      }
       double2 v[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] = make_double2(XO[0][src[q]], XO[1][src[q]]);
+      for (int t = 0; t < 4; ++t) {
+        const int32_t i = i0 + 64 * t < total ? i0 + 64 * t : total - 1;
+        const int32_t b = i + (i >= u.uo[2] ? d2 : (i >= u.uo[1] ? d1 : d0));
+        v[t] = make_double2(XN[0][b], XN[1][b]);
+      }
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        if (i0 + 64 * q < total) wl[i0 + 64 * q] = v[q];
+      for (int t = 0; t < 4; ++t)
+        if (i0 + 64 * t < total) wl[i0 + 64 * t] = v[t];
     }
     // the wave's LDS operations complete in issue order; the fence keeps the
     // compiler from moving the reads below ahead of these writes
@@ -396,7 +368,7 @@ __device__ __forceinline__ void nmask2_perm(const SphBufs<R>& B, const SphConst&
     for (int k = 0; k < 3; ++k) {
       for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
         const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
-        const uint32_t mk = window_bits_perm(XO[0], XO[1], perm, B.n, xa[0], xa[1], a, w0, hi, dist2);
+        const uint32_t mk = window_bits<2>(XN[0], XN[1], XN[1], xa[0], xa[1], 0.0, a, w0, hi, dist2);
         if (mk != 0) {
           if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
           ++cnt;
@@ -421,7 +393,7 @@ __global__ __launch_bounds__(256) void k_reorder_nmask2(SS ss) {
   __shared__ double2 lds[kBlock / 64][kNmaskWave + 4];  // +4: window reads run up to 3 past a row's end
   double xa[3] = {0.0, 0.0, 0.0};
   if (a < B.n) reorder_one<2, R>(B, C, G, F, ctl, a, xa);
-  nmask2_perm<R>(B, C, G, ctl, a, a0, xa, lds);
+  nmask2_new<R>(B, C, G, ctl, a, a0, xa, lds);
 }
 
 // ---- corrections after the update (src/sphproblem.jl:303-322) --------------
@@ -1018,12 +990,20 @@ __global__ void kb_scatter(SS ss) {
   sd.tmp[sd.start[sd.key[a]] + sd.slot[a]] = int32_t(a);
 }
 
-// Rank inside the cell by input index: the final order is (cell, input).
+// Rank inside the cell by input index: the final order is (cell, input).  The
+// particle's position moves to its new place here (loaded with the key, so it
+// arrives while the rank is counted), so the membership store after it reads
+// the candidates of a stencil row as contiguous ranges of the new position
+// arrays instead of through the permutation.
 template <int D, typename R, typename SS>
 __global__ void kb_rank(SS ss) {
   GCLSPH_SCENE(ss);
   if (B.ctl->rebuild == 0 || a >= B.n) return;
   const int32_t i = sd.tmp[a];
+  const int p = B.ctl->parity;
+  double x[3];
+#pragma unroll
+  for (int k = 0; k < D; ++k) x[k] = B.X[p][k][i];
   const int32_t c = sd.key[i];
   const int32_t b = sd.start[c], e = sd.start[c + 1];
   // four loads in flight per trip (indices clamped into the cell, which holds
@@ -1038,6 +1018,8 @@ __global__ void kb_rank(SS ss) {
   }
   sd.perm[b + r] = i;
   sd.skey[b + r] = c;
+#pragma unroll
+  for (int k = 0; k < D; ++k) B.X[p ^ 1][k][b + r] = x[k];
 }
 
 // ------------------------------------------------------------ host launchers
