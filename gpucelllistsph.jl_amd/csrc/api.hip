@@ -157,10 +157,9 @@ struct gcl_s {
 
   // update! (src/celllist.jl:95-134) for a standalone cell list.
   void update(bool fillzero) {
-    bin_sort(G, prec, xo, n, false, sb, nullptr, stream);
+    bin_sort(G, prec, xo, n, false, sb, nullptr, stream, nullptr, nullptr, xs);
     int32_t maxpoint = read_maxcell();
     if (maxpoint > mppcell) mppcell = int64_t(std::ceil(double(maxpoint) * 1.05 + 1));
-    gather_positions(dim, prec, xo, xs, sb.perm, n, stream);
     bool fresh = false;  // a reallocated buffer is zero-filled whole
     if (pairsn > pb.capacity || double(pairsn) < double(pb.capacity) * 0.6) {
       pb.set_capacity(ceil_div08(pairsn), stream);
@@ -1109,11 +1108,10 @@ int gcl_create(int dim, int prec, int64_t n, const double* const* pts, const dou
     g->sb.alloc(n, g->G.ncells());
     g->pb.alloc(n, dim == 2 && n <= kPairEntriesMaxPoints ? kPairEntries2D : 0);
     // cellmap! with the Float64 MIN (:55), then fillcells_naive! (:66)
-    bin_sort(g->G, prec, g->xo, n, true, g->sb, nullptr, g->stream);
+    bin_sort(g->G, prec, g->xo, n, true, g->sb, nullptr, g->stream, nullptr, nullptr, g->xs);
     int32_t maxc = g->read_maxcell();
     int64_t maxpoint = int64_t(std::ceil(double(maxc) * 1.05 + 1));  // :57-58
     g->mppcell = mppcell < maxpoint ? maxpoint : mppcell;
-    gather_positions(dim, prec, g->xo, g->xs, g->sb.perm, n, g->stream);
     if (mpairs == 0) {  // мaxpairs (:69-71)
       unsigned long long* d = nullptr;
       GCL_HIP(hipMalloc(&d, 8));

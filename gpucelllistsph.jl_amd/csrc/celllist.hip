@@ -68,13 +68,23 @@ __global__ void k_scatter(const int32_t* __restrict__ key, const int32_t* __rest
 }
 
 // Rank each item inside its cell by input index: final order is (cell, input).
+// Pos: the item's position also moves to its sorted place in xs (loaded with
+// the key, so it arrives while the rank is counted), which replaces a
+// separate gather of the positions.
+template <typename T, int D, bool Pos>
 __global__ void k_rank(const int32_t* __restrict__ key, const int32_t* __restrict__ start,
                        const int32_t* __restrict__ tmp, int32_t* __restrict__ perm, int32_t* __restrict__ skey,
-                       int64_t n, const int* flag) {
+                       int64_t n, const int* flag, XPtrs xp, XPtrs xalt, const int* sel, XPtrs xsp) {
   if (flag && *flag == 0) return;
   int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
   int32_t i = tmp[s];
+  T x[3];
+  if constexpr (Pos) {
+    const XPtrs& src = (sel && *sel) ? xalt : xp;
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = static_cast<const T*>(src.p[k])[i];
+  }
   int32_t c = key[i];
   int32_t b = start[c], e = start[c + 1];
   // four loads in flight per trip (indices clamped into the cell, which holds
@@ -89,6 +99,10 @@ __global__ void k_rank(const int32_t* __restrict__ key, const int32_t* __restric
   }
   perm[b + r] = i;
   skey[b + r] = c;
+  if constexpr (Pos) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) static_cast<T*>(const_cast<void*>(xsp.p[k]))[b + r] = x[k];
+  }
 }
 
 template <typename T, int D>
@@ -405,7 +419,7 @@ __global__ __launch_bounds__(256) void k_pair_emit_wave(XPtrs xsp, const int32_t
 
 template <typename T>
 void bin_sort_t(const CellGeom& g, const void* const* x, int64_t n, bool use_off64, SortBuffers& sb, const int* flag,
-                hipStream_t s, const void* const* x_alt, const int* sel) {
+                hipStream_t s, const void* const* x_alt, const int* sel, void* const* xs) {
   XPtrs xp{{x[0], x[1], g.dim == 3 ? x[2] : x[1]}};
   XPtrs xa = xp;
   if (x_alt) xa = XPtrs{{x_alt[0], x_alt[1], g.dim == 3 ? x_alt[2] : x_alt[1]}};
@@ -422,7 +436,16 @@ void bin_sort_t(const CellGeom& g, const void* const* x, int64_t n, bool use_off
   exclusive_scan_i32(sb.cnt, sb.start, nc, sb.scan_tmp, flag, s, sb.maxcell);
   k_scatter<<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.slot, sb.start, sb.tmp, n, flag);
   GCL_HIP(hipGetLastError());
-  k_rank<<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.start, sb.tmp, sb.perm, sb.skey, n, flag);
+  const XPtrs xsp = xs ? XPtrs{{xs[0], xs[1], g.dim == 3 ? xs[2] : xs[1]}} : XPtrs{{nullptr, nullptr, nullptr}};
+  if (!xs)
+    k_rank<T, 2, false><<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.start, sb.tmp, sb.perm, sb.skey, n, flag, xp, xa, sel,
+                                                        xsp);
+  else if (g.dim == 2)
+    k_rank<T, 2, true><<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.start, sb.tmp, sb.perm, sb.skey, n, flag, xp, xa, sel,
+                                                       xsp);
+  else
+    k_rank<T, 3, true><<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.start, sb.tmp, sb.perm, sb.skey, n, flag, xp, xa, sel,
+                                                       xsp);
   GCL_HIP(hipGetLastError());
   if (!flag) sb.sorted = true;  // perm now holds a permutation of [0, n) (a flagged sort may not have run)
 }
@@ -496,9 +519,9 @@ void PairBuffers::release() {
 }
 
 void bin_sort(const CellGeom& g, int prec, const void* const* x, int64_t n, bool use_off64, SortBuffers& sb,
-              const int* flag, hipStream_t s, const void* const* x_alt, const int* sel) {
-  if (prec == 32) bin_sort_t<float>(g, x, n, use_off64, sb, flag, s, x_alt, sel);
-  else bin_sort_t<double>(g, x, n, use_off64, sb, flag, s, x_alt, sel);
+              const int* flag, hipStream_t s, const void* const* x_alt, const int* sel, void* const* xs) {
+  if (prec == 32) bin_sort_t<float>(g, x, n, use_off64, sb, flag, s, x_alt, sel, xs);
+  else bin_sort_t<double>(g, x, n, use_off64, sb, flag, s, x_alt, sel, xs);
 }
 
 void gather_positions(int dim, int prec, const void* const* x, void* const* xs, const int32_t* perm, int64_t n,

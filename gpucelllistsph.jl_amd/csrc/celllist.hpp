@@ -191,8 +191,11 @@ __device__ __forceinline__ void row_union(RowUnion& u, const int32_t* start, con
 // `prec` is 64 or 32; `use_off64` selects the Float64-offset cell map.
 // `x_alt`/`sel` (nullable): a second set of position pointers, chosen on the
 // device when *sel != 0 (the SPH state is double-buffered on the device).
+// `xs` (nullable): D arrays that receive the positions in sorted order (what
+// gather_positions would write), moved by the sort's rank kernel.
 void bin_sort(const CellGeom& g, int prec, const void* const* x, int64_t n, bool use_off64, SortBuffers& sb,
-              const int* flag, hipStream_t s, const void* const* x_alt = nullptr, const int* sel = nullptr);
+              const int* flag, hipStream_t s, const void* const* x_alt = nullptr, const int* sel = nullptr,
+              void* const* xs = nullptr);
 
 // Gathers x (input order) into xs (sorted order) using perm.
 void gather_positions(int dim, int prec, const void* const* x, void* const* xs, const int32_t* perm, int64_t n,

@@ -155,8 +155,7 @@ extern "C" int gcl_neighborlist_noalloc(int dim, int64_t n, const double* const*
 
     // cellmap! + fillcells_naive! with the Float64 MIN, as the constructor
     // bins (src/celllist.jl:55-66), then the pair search (:72-75)
-    bin_sort(g, GCLSPH_F64, x, n, true, sb, nullptr, s);
-    gather_positions(dim, GCLSPH_F64, x, xs, sb.perm, n, s);
+    bin_sort(g, GCLSPH_F64, x, n, true, sb, nullptr, s, nullptr, nullptr, xs);
     pair_count(g, GCLSPH_F64, xs, sb.start, sb.skey, n, dist, pb, s);
     int64_t total = 0;
     GCL_HIP(hipMemcpyAsync(&total, pb.off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
