@@ -35,6 +35,15 @@ struct XPtrs {
   const void* p[3];
 };
 
+// One (i, j) pair of the output list, stored past the caches' normal
+// allocation (non-temporal): the list is written once here and read later by
+// the caller, so it should not push the binning and search data out of L2.
+__device__ __forceinline__ void store_pair(int2* out, int64_t o, int32_t a, int32_t b) {
+  using v2i = int __attribute__((ext_vector_type(2)));
+  const v2i v = {a < b ? a : b, a < b ? b : a};
+  __builtin_nontemporal_store(v, reinterpret_cast<v2i*>(out + o));
+}
+
 template <typename T, int D>
 __global__ void k_cellmap(XPtrs xp, XPtrs xalt, const int* sel, int64_t n, CellGeom g, int use_off64,
                           int32_t* __restrict__ key, int32_t* __restrict__ slot, int32_t* __restrict__ cnt,
@@ -314,7 +323,7 @@ __global__ __launch_bounds__(256) void k_pair_emit(XPtrs xsp, const int32_t* __r
           st[o - o0] = make_int2(is, t);
         } else {
           const int32_t it = id[t] + 1;
-          out[o] = is < it ? make_int2(is, it) : make_int2(it, is);
+          store_pair(out, o, is, it);
         }
       }
     };
@@ -355,14 +364,22 @@ __global__ __launch_bounds__(256) void k_pair_emit(XPtrs xsp, const int32_t* __r
   // and an odd last pair go out on their own
   const int lane = int(threadIdx.x & 63);
   const int64_t h = (reinterpret_cast<uintptr_t>(out + o0) & 15) != 0 && ns > 0 ? 1 : 0;
-  if (h && lane == 0) out[o0] = pair_at(0);
+  if (h && lane == 0) {
+    const int2 q = pair_at(0);
+    store_pair(out, o0, q.x, q.y);
+  }
   const int64_t body = (ns - h) / 2;
   int4* out4 = reinterpret_cast<int4*>(out + o0 + h);
   for (int64_t j = lane; j < body; j += 64) {
     const int2 p0 = pair_at(h + 2 * j), p1 = pair_at(h + 2 * j + 1);
-    out4[j] = make_int4(p0.x, p0.y, p1.x, p1.y);
+    using v4i = int __attribute__((ext_vector_type(4)));
+    const v4i v = {p0.x, p0.y, p1.x, p1.y};
+    __builtin_nontemporal_store(v, reinterpret_cast<v4i*>(out4 + j));  // the list is not read back here
   }
-  if (h + 2 * body < ns && lane == 0) out[o0 + ns - 1] = pair_at(ns - 1);
+  if (h + 2 * body < ns && lane == 0) {
+    const int2 q = pair_at(ns - 1);
+    store_pair(out, o0 + ns - 1, q.x, q.y);
+  }
 }
 
 // Dense lists: one wave per sorted item s.  The 64 lanes test 64 consecutive
@@ -409,7 +426,7 @@ __global__ __launch_bounds__(256) void k_pair_emit_wave(XPtrs xsp, const int32_t
             o + __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
         if (slot < capacity) {
           const int32_t it = id[j] + 1;
-          out[slot] = is < it ? make_int2(is, it) : make_int2(it, is);
+          store_pair(out, slot, is, it);
         }
       }
       o += __builtin_popcountll(m);
