@@ -363,6 +363,7 @@ struct sph_s {
     };
     C.wnorm = norm * ipow(C.Hinv, dim);
     C.dwnorm = norm * ipow(C.Hinv, dim + 1);
+    C.dist = sys->dist;
     C.dist2 = sys->dist * sys->dist;
     C.fb_d[0] = 0.4;
     C.fb_r0[0] = 2 * C.dx;

@@ -21,7 +21,7 @@ struct SphConst {
   double DDTgz;         // ρ0 g / Cb               :560
   double P0;            // c0^2 ρ0 / γ             :712
   double wnorm, dwnorm; // σ_D H⁻ᴰ, σ_D H⁻⁽ᴰ⁺¹⁾ (WendlandC2)
-  double dist2;         // neighbour-list cutoff dist^2 (src/gpukernels_cls.jl:294)
+  double dist, dist2;   // neighbour-list cutoff and dist^2 (src/gpukernels_cls.jl:294)
   double fb_d[2], fb_r0[2], fb_eta2[2];  // fbmolforce (0.4, 2dx) / (bound_D, bound_l); η² = 0.01 r0^2
   double st_2h, st_c, st_01h;            // 2h, 1.5π, 0.1h       :1393-1394
   double dpc_l0, dpc_l0inv, dpc_eta2, dpc_whinv, dpc_lim, dpc_wnorm, dpc_lambda, dpc_pmin, dpc_pmax;

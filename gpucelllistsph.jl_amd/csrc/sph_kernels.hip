@@ -93,6 +93,41 @@ __device__ __forceinline__ uint32_t window_bits_lds3(const double* lx, const dou
   return window_mask(acc, len, a, lo);
 }
 
+// The candidates of a stencil row that can pass the membership test, in 3D
+// where the rank orders every cell by x (so a row, three consecutive cells of
+// one y and z, is in x order): [lo, hi) narrowed by bisection over the staged
+// x (lx[b] for candidate b) to the first b with xa - x_b < dist and, from
+// there, the first b with xa - x_b <= -dist.  Every candidate outside has
+// |fl(xa - x_b)| >= dist, the same FP64 difference the test squares, so its
+// r^2 rounds to at least fl(dist^2) = dist2 and the test fails.  A NaN
+// coordinate (never a member: its r^2 is NaN) compares false both ways; the
+// bisection still only drops candidates whose real x lies beyond an x it
+// found outside the range, since the real x are in order.
+__device__ __forceinline__ void x_range(const double* lx, double xa, double dist, int32_t& lo, int32_t& hi) {
+  int32_t n = hi - lo;
+  while (n > 0) {
+    const int32_t h = n >> 1;
+    if (xa - lx[lo + h] >= dist) {
+      lo += h + 1;
+      n -= h + 1;
+    } else {
+      n = h;
+    }
+  }
+  int32_t b = lo;
+  n = hi - lo;
+  while (n > 0) {
+    const int32_t h = n >> 1;
+    if (xa - lx[b + h] <= -dist) {
+      n = h;
+    } else {
+      b += h + 1;
+      n -= h + 1;
+    }
+  }
+  hi = b;
+}
+
 // Candidates staged in LDS: 2D 400 positions per wave (the union of a wave's
 // three stencil rows holds ~300 on a uniform lattice), 26 KiB per block, six
 // blocks per CU; 3D 1280 per block and plane of three stencil rows, 32 KiB.
@@ -112,9 +147,10 @@ constexpr int kNmaskPlane = 1280;
 // k_nmask is the 3D store (the 2D one runs inside k_reorder_nmask2 below):
 // the block stages the candidates' positions plane by plane (three stencil
 // rows at a time, ~1000 candidates per particle in all) and every lane tests
-// its candidates there; a block whose union exceeds the LDS budget reads them
-// from global memory.  The tests are the same expression in every path, so
-// the bits are the same.
+// there the candidates of each row within dist of it in x (x_range, ~46 % of
+// them); a block whose union exceeds the LDS budget reads the whole rows from
+// global memory.  The tests are the same expression in every path, so the
+// bits are the same.
 template <int D, typename R, typename SS>
 __global__ __launch_bounds__(256) void k_nmask(SS ss) {
   static_assert(D == 3, "the 2D membership store is part of k_reorder_nmask2");
@@ -185,34 +221,56 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
                                            dist2)
                         : window_bits<D>(XM[0], XM[1], XM[2], xa0, xa1, xa2, a, w0, hi, dist2);
         };
-        if constexpr (cell_windows<D, R>()) {
-#pragma unroll
-          for (int j = 0; j < 3; ++j) {  // the row's cells ox = j - 1, in sorted order
-            const int64_t x = ix + j - 1;
-            if (x < 0 || x >= gx) continue;
-            const int32_t cb = sb[kk][j], ce = sb[kk][j + 1];
-            const uint64_t code = uint64_t(win_code3(j - 1, kk - 1, pl - 1)) << kWin3;
-            for (int32_t w0 = cb; w0 < ce; w0 += kWin3) {
-              const int32_t hi = ce < w0 + kWin3 ? ce : w0 + kWin3;
-              const uint32_t mk = test(w0, hi);
-              if (mk != 0) {
-                if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | code | mk;
-                ++cnt;
-              }
-            }
+        // the row's candidates in the grid, narrowed to |x_a - x_b| < dist
+        // where they are staged, tested in 32-candidate windows (one trip
+        // count for every lane of the wave, whatever its x in its cell)
+        int32_t rb = ix >= 1 ? sb[kk][0] : sb[kk][1];
+        int32_t re = ix + 1 < gx ? sb[kk][3] : sb[kk][2];
+        if (staged) x_range(lx + off, xa0, C.dist, rb, re);
+        // in 3D FP32-split the member bits are stored as windows of one cell
+        // (the cell frames of the records need it), merged across the test
+        // windows: from a first member, up to kWin3 candidates of its cell
+        // (ox = pj - 1); pm holds the pending window's bits
+        int32_t pb = 0;
+        int pj = 0;
+        uint32_t pm = 0;
+        auto flush = [&]() {
+          if (pm != 0) {
+            const uint64_t code = uint64_t(win_code3(pj - 1, kk - 1, pl - 1)) << kWin3;
+            if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(pb)) << 32) | code | pm;
+            ++cnt;
+            pm = 0;
           }
-        } else {  // 32-candidate windows over the whole row
-          const int32_t rb = ix >= 1 ? sb[kk][0] : sb[kk][1];
-          const int32_t re = ix + 1 < gx ? sb[kk][3] : sb[kk][2];
-          for (int32_t w0 = rb; w0 < re; w0 += 32) {
-            const int32_t hi = re < w0 + 32 ? re : w0 + 32;
-            const uint32_t mk = test(w0, hi);
+        };
+        for (int32_t w0 = rb; w0 < re; w0 += 32) {
+          const int32_t hi = re < w0 + 32 ? re : w0 + 32;
+          const uint32_t mk = test(w0, hi);
+          if constexpr (cell_windows<D, R>()) {
+            uint64_t rest = mk;
+            while (rest != 0) {
+              const int t0 = __builtin_ctzll(rest);
+              const int32_t b0 = w0 + t0;
+              const int j = (b0 >= sb[kk][1]) + (b0 >= sb[kk][2]);
+              if (pm == 0 || j != pj || b0 >= pb + int32_t(kWin3)) {
+                flush();
+                pb = b0;
+                pj = j;
+              }
+              // this test window's bits from b0 to the end of the pending
+              // window or of the cell, whichever comes first (1..kWin3 of them)
+              const int32_t ce = sb[kk][j + 1], we = pb + int32_t(kWin3);
+              const uint64_t lm = (uint64_t(1) << ((ce < we ? ce : we) - b0)) - 1u;
+              pm |= uint32_t(((rest >> t0) & lm) << (b0 - pb));
+              rest &= ~(lm << t0);
+            }
+          } else {
             if (mk != 0) {
               if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
               ++cnt;
             }
           }
         }
+        if constexpr (cell_windows<D, R>()) flush();
       }
     }
     __syncthreads();  // the next plane overwrites the staged positions
@@ -983,14 +1041,29 @@ __global__ void kb_tile_scan(SS ss) {
   if (m.y == 0 && threadIdx.x == 0) sd.start[n] = int32_t(sd.scan_tmp[nt]);
 }
 
+// Sort key of an x coordinate: its bit pattern made unsigned-monotone (order
+// of the key = order of the value for every non-NaN x; -0 before +0).
+__device__ __forceinline__ uint64_t x_order_key(double x) {
+  const uint64_t u = uint64_t(__double_as_longlong(x));
+  return (u >> 63) ? ~u : (u | (uint64_t(1) << 63));
+}
+
+// Arrival slot -> input index; in 3D also the x key of the input at its slot
+// (TMP, free during the rebuild), for the rank below.
 template <int D, typename R, typename SS>
 __global__ void kb_scatter(SS ss) {
   GCLSPH_SCENE(ss);
   if (B.ctl->rebuild == 0 || a >= B.n) return;
-  sd.tmp[sd.start[sd.key[a]] + sd.slot[a]] = int32_t(a);
+  const int32_t s = sd.start[sd.key[a]] + sd.slot[a];
+  sd.tmp[s] = int32_t(a);
+  if constexpr (D == 3) reinterpret_cast<uint64_t*>(B.TMP)[s] = x_order_key(B.X[B.ctl->parity][0][a]);
 }
 
-// Rank inside the cell by input index: the final order is (cell, input).  The
+// Rank inside the cell: by input index in 2D, the final order is (cell,
+// input); by (x, input) in 3D, so that the candidates of a stencil row (three
+// consecutive cells of one y, z) are in x order and the membership store can
+// bound each particle's candidates in a row to |x_a - x_b| < dist by bisection
+// (the cell of a coordinate is a non-decreasing function of it).  The
 // particle's position moves to its new place here (loaded with the key, so it
 // arrives while the rank is counted), so the membership store after it reads
 // the candidates of a stencil row as contiguous ranges of the new position
@@ -1009,12 +1082,29 @@ __global__ void kb_rank(SS ss) {
   // four loads in flight per trip (indices clamped into the cell, which holds
   // at least this particle; the clamped ones are not counted)
   int32_t r = 0;
-  for (int32_t k = b; k < e; k += 4) {
-    int32_t v[4];
+  if constexpr (D == 3) {
+    const uint64_t* K = reinterpret_cast<const uint64_t*>(B.TMP);
+    const uint64_t ki = K[a];
+    for (int32_t k = b; k < e; k += 4) {
+      int32_t v[4];
+      uint64_t kv[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = sd.tmp[k + u < e ? k + u : e - 1];
+      for (int u = 0; u < 4; ++u) {
+        const int32_t j = k + u < e ? k + u : e - 1;
+        v[u] = sd.tmp[j];
+        kv[u] = K[j];
+      }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) r += (k + u < e) & (v[u] < i);
+      for (int u = 0; u < 4; ++u) r += (k + u < e) & ((kv[u] < ki) | ((kv[u] == ki) & (v[u] < i)));
+    }
+  } else {
+    for (int32_t k = b; k < e; k += 4) {
+      int32_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = sd.tmp[k + u < e ? k + u : e - 1];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) r += (k + u < e) & (v[u] < i);
+    }
   }
   sd.perm[b + r] = i;
   sd.skey[b + r] = c;
