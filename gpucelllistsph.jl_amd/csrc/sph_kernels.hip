@@ -93,10 +93,10 @@ __device__ __forceinline__ uint32_t window_bits_lds3(const double* lx, const dou
   return window_mask(acc, len, a, lo);
 }
 
-// The candidates of a stencil row that can pass the membership test, in 3D
-// where the rank orders every cell by x (so a row, three consecutive cells of
-// one y and z, is in x order): [lo, hi) narrowed by bisection over the staged
-// x (lx[b] for candidate b) to the first b with xa - x_b < dist and, from
+// The candidates of a stencil row that can pass the membership test, where
+// the rank orders every cell by x (so a row, three consecutive cells of one
+// y and z, is in x order): [lo, hi) narrowed by bisection over the staged
+// x (lx[b] of candidate b) to the first b with xa - x_b < dist and, from
 // there, the first b with xa - x_b <= -dist.  Every candidate outside has
 // |fl(xa - x_b)| >= dist, the same FP64 difference the test squares, so its
 // r^2 rounds to at least fl(dist^2) = dist2 and the test fails.  A NaN
@@ -128,6 +128,40 @@ __device__ __forceinline__ void x_range(const double* lx, double xa, double dist
   hi = b;
 }
 
+// Row windows as stored: up to 32 candidates from a first member, the member
+// bits of the consecutive test windows of one row merged into the pending
+// window while they fall inside it (fewer, fuller windows for the pair
+// loops).  add() in row order, flush() at the end of the row.
+struct RowWindows {
+  uint64_t* out;
+  int64_t n;
+  int cap;
+  int& cnt;
+  int32_t pb = 0;
+  uint32_t pm = 0;
+  __device__ __forceinline__ void flush() {
+    if (pm != 0) {
+      if (cnt < cap) out[int64_t(cnt) * n] = (uint64_t(uint32_t(pb)) << 32) | pm;
+      ++cnt;
+      pm = 0;
+    }
+  }
+  __device__ __forceinline__ void add(int32_t w0, uint32_t mk) {
+    uint64_t rest = mk;
+    while (rest != 0) {
+      const int t0 = __builtin_ctzll(rest);
+      const int32_t b0 = w0 + t0;
+      if (pm == 0 || b0 >= pb + 32) {
+        flush();
+        pb = b0;
+      }
+      const uint64_t lm = (uint64_t(1) << (pb + 32 - b0)) - 1u;  // 1..32 bits
+      pm |= uint32_t(((rest >> t0) & lm) << (b0 - pb));
+      rest &= ~(lm << t0);
+    }
+  }
+};
+
 // Candidates staged in LDS: 2D 400 positions per wave (the union of a wave's
 // three stencil rows holds ~300 on a uniform lattice), 26 KiB per block, six
 // blocks per CU; 3D 1280 per block and plane of three stencil rows, 32 KiB.
@@ -136,11 +170,13 @@ constexpr int kNmaskPlane = 1280;
 
 // Neighbour-list membership of the last rebuild, stored per particle as
 // (mask, base) entries: one window of candidates [base, base + len) with its
-// member bits in the low word.  32-candidate windows of one stencil row, or in
-// 3D FP32-split windows of up to kWin3 candidates inside one stencil cell with
-// the cell's offset coded above the member bits (win_code3), which is what
-// the cell frames of its partner records need.  Only windows with members are
-// stored, in stencil order.  Entry e of particle a is NE[e * n + a]; NC[a]
+// member bits in the low word.  Windows of up to 32 candidates of one
+// stencil row, or in 3D FP32-split of up to kWin3 candidates inside one
+// stencil cell with the cell's offset coded above the member bits
+// (win_code3), which is what the cell frames of its partner records need.
+// Each window starts at a member (the first one not covered by the window
+// before it), so with the rows in x order a particle's partners fill few
+// windows; stored in stencil order.  Entry e of particle a is NE[e * n + a]; NC[a]
 // is the entry count, or -1 when the particle needs more than necap entries
 // (its partners are then found by scanning the stencil).
 //
@@ -234,6 +270,7 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
         int32_t pb = 0;
         int pj = 0;
         uint32_t pm = 0;
+        RowWindows rw{out, B.n, cap, cnt};  // FP64: 32-candidate row windows
         auto flush = [&]() {
           if (pm != 0) {
             const uint64_t code = uint64_t(win_code3(pj - 1, kk - 1, pl - 1)) << kWin3;
@@ -264,13 +301,11 @@ __global__ __launch_bounds__(256) void k_nmask(SS ss) {
               rest &= ~(lm << t0);
             }
           } else {
-            if (mk != 0) {
-              if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
-              ++cnt;
-            }
+            rw.add(w0, mk);
           }
         }
         if constexpr (cell_windows<D, R>()) flush();
+        else rw.flush();
       }
     }
     __syncthreads();  // the next plane overwrites the staged positions
@@ -410,28 +445,26 @@ __device__ __forceinline__ void nmask2_new(const SphBufs<R>& B, const SphConst& 
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const double2* lrow = wl + (u.uo[k] - u.ub[k]);  // candidate b at lrow[b]
+        // (a row of ~35 candidates: bounding it to |x_a - x_b| < dist by
+        // bisection, as in 3D, costs more than the tests it saves)
+        RowWindows rw{out, B.n, cap, cnt};
         for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
           const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
-          const uint32_t mk = window_bits_lds(lrow + w0, xa[0], xa[1], a, w0, hi, dist2);
-          if (mk != 0) {
-            if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
-            ++cnt;
-          }
+          rw.add(w0, window_bits_lds(lrow + w0, xa[0], xa[1], a, w0, hi, dist2));
         }
+        rw.flush();
       }
     }
   } else if (active) {
     const Rows<2> rows = stencil_rows<2>(G, B.start, B.skey[a]);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
+      RowWindows rw{out, B.n, cap, cnt};
       for (int32_t w0 = rows.b[k]; w0 < rows.e[k]; w0 += 32) {
         const int32_t hi = rows.e[k] < w0 + 32 ? rows.e[k] : w0 + 32;
-        const uint32_t mk = window_bits<2>(XN[0], XN[1], XN[1], xa[0], xa[1], 0.0, a, w0, hi, dist2);
-        if (mk != 0) {
-          if (cnt < cap) out[int64_t(cnt) * B.n] = (uint64_t(uint32_t(w0)) << 32) | mk;
-          ++cnt;
-        }
+        rw.add(w0, window_bits<2>(XN[0], XN[1], XN[1], xa[0], xa[1], 0.0, a, w0, hi, dist2));
       }
+      rw.flush();
     }
   }
   if (active) B.NC[a] = cnt <= cap ? cnt : -1;
@@ -1048,43 +1081,75 @@ __device__ __forceinline__ uint64_t x_order_key(double x) {
   return (u >> 63) ? ~u : (u | (uint64_t(1) << 63));
 }
 
-// Arrival slot -> input index; in 3D also the x key of the input at its slot
-// (TMP, free during the rebuild), for the rank below.
+// Arrival slot -> input index, and the x key of the input at its slot (TMP,
+// free during the rebuild), for the rank below.
 template <int D, typename R, typename SS>
 __global__ void kb_scatter(SS ss) {
   GCLSPH_SCENE(ss);
   if (B.ctl->rebuild == 0 || a >= B.n) return;
   const int32_t s = sd.start[sd.key[a]] + sd.slot[a];
   sd.tmp[s] = int32_t(a);
-  if constexpr (D == 3) reinterpret_cast<uint64_t*>(B.TMP)[s] = x_order_key(B.X[B.ctl->parity][0][a]);
+  reinterpret_cast<uint64_t*>(B.TMP)[s] = x_order_key(B.X[B.ctl->parity][0][a]);
 }
 
-// Rank inside the cell: by input index in 2D, the final order is (cell,
-// input); by (x, input) in 3D, so that the candidates of a stencil row (three
-// consecutive cells of one y, z) are in x order and the membership store can
-// bound each particle's candidates in a row to |x_a - x_b| < dist by bisection
-// (the cell of a coordinate is a non-decreasing function of it).  The
-// particle's position moves to its new place here (loaded with the key, so it
-// arrives while the rank is counted), so the membership store after it reads
-// the candidates of a stencil row as contiguous ranges of the new position
-// arrays instead of through the permutation.
+// Rank inside the cell by (x, input index): the final order is (cell, x,
+// input), so the candidates of a stencil row (three consecutive cells of one
+// y, and z) are in x order (the cell of a coordinate is a non-decreasing
+// function of it): a particle's partners in a row are then near each other,
+// so they fill fewer stored windows, and in 3D the membership store bounds
+// each particle's candidates in a row to |x_a - x_b| < dist by bisection.
+// The order is the same for every run (ties in x by input index).  The
+// particle's
+// position moves to its new place here (loaded with the key, so it arrives
+// while the rank is counted), so the membership store after it reads the
+// candidates of a stencil row as contiguous ranges of the new position arrays
+// instead of through the permutation.
 template <int D, typename R, typename SS>
-__global__ void kb_rank(SS ss) {
+__global__ __launch_bounds__(256) void kb_rank(SS ss) {
   GCLSPH_SCENE(ss);
-  if (B.ctl->rebuild == 0 || a >= B.n) return;
-  const int32_t i = sd.tmp[a];
+  if (B.ctl->rebuild == 0) return;
+  const int64_t a0 = int64_t(m.y) * kBlock;
+  if (a0 >= B.n) return;  // whole block
+  // the (key, input) of the block's slots, which hold the whole cell of most
+  // of its particles, are read once and compared from LDS
+  __shared__ uint64_t lk[kBlock];
+  __shared__ int32_t li[kBlock];
+  const uint64_t* K = reinterpret_cast<const uint64_t*>(B.TMP);
+  const bool active = a < B.n;
+  int32_t i = 0;
+  uint64_t ki = 0;
+  if (active) {
+    i = sd.tmp[a];
+    ki = K[a];
+    lk[threadIdx.x] = ki;
+    li[threadIdx.x] = i;
+  }
+  __syncthreads();
+  if (!active) return;
   const int p = B.ctl->parity;
   double x[3];
 #pragma unroll
   for (int k = 0; k < D; ++k) x[k] = B.X[p][k][i];
   const int32_t c = sd.key[i];
   const int32_t b = sd.start[c], e = sd.start[c + 1];
-  // four loads in flight per trip (indices clamped into the cell, which holds
-  // at least this particle; the clamped ones are not counted)
+  const int32_t s0 = int32_t(a0), s1 = int32_t(a0 + kBlock < B.n ? a0 + kBlock : B.n);
+  // four compares per trip (indices clamped into the cell, which holds at
+  // least this particle; the clamped ones are not counted)
   int32_t r = 0;
-  if constexpr (D == 3) {
-    const uint64_t* K = reinterpret_cast<const uint64_t*>(B.TMP);
-    const uint64_t ki = K[a];
+  if (b >= s0 && e <= s1) {
+    for (int32_t k = b; k < e; k += 4) {
+      int32_t v[4];
+      uint64_t kv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int32_t j = (k + u < e ? k + u : e - 1) - s0;
+        v[u] = li[j];
+        kv[u] = lk[j];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) r += (k + u < e) & ((kv[u] < ki) | ((kv[u] == ki) & (v[u] < i)));
+    }
+  } else {
     for (int32_t k = b; k < e; k += 4) {
       int32_t v[4];
       uint64_t kv[4];
@@ -1096,14 +1161,6 @@ __global__ void kb_rank(SS ss) {
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) r += (k + u < e) & ((kv[u] < ki) | ((kv[u] == ki) & (v[u] < i)));
-    }
-  } else {
-    for (int32_t k = b; k < e; k += 4) {
-      int32_t v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = sd.tmp[k + u < e ? k + u : e - 1];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) r += (k + u < e) & (v[u] < i);
     }
   }
   sd.perm[b + r] = i;
