@@ -1081,15 +1081,11 @@ __device__ __forceinline__ uint64_t x_order_key(double x) {
   return (u >> 63) ? ~u : (u | (uint64_t(1) << 63));
 }
 
-// Arrival slot -> input index, and the x key of the input at its slot (TMP,
-// free during the rebuild), for the rank below.
 template <int D, typename R, typename SS>
 __global__ void kb_scatter(SS ss) {
   GCLSPH_SCENE(ss);
   if (B.ctl->rebuild == 0 || a >= B.n) return;
-  const int32_t s = sd.start[sd.key[a]] + sd.slot[a];
-  sd.tmp[s] = int32_t(a);
-  reinterpret_cast<uint64_t*>(B.TMP)[s] = x_order_key(B.X[B.ctl->parity][0][a]);
+  sd.tmp[sd.start[sd.key[a]] + sd.slot[a]] = int32_t(a);
 }
 
 // Rank inside the cell by (x, input index): the final order is (cell, x,
@@ -1110,26 +1106,25 @@ __global__ __launch_bounds__(256) void kb_rank(SS ss) {
   if (B.ctl->rebuild == 0) return;
   const int64_t a0 = int64_t(m.y) * kBlock;
   if (a0 >= B.n) return;  // whole block
-  // the (key, input) of the block's slots, which hold the whole cell of most
-  // of its particles, are read once and compared from LDS
+  // the (x key, input) of the block's slots, which hold the whole cell of
+  // most of its particles, are shared through LDS; a cell across the block's
+  // edge reads its members' x from global memory
   __shared__ uint64_t lk[kBlock];
   __shared__ int32_t li[kBlock];
-  const uint64_t* K = reinterpret_cast<const uint64_t*>(B.TMP);
+  const int p = B.ctl->parity;
   const bool active = a < B.n;
   int32_t i = 0;
-  uint64_t ki = 0;
+  double x[3] = {0.0, 0.0, 0.0};
   if (active) {
     i = sd.tmp[a];
-    ki = K[a];
-    lk[threadIdx.x] = ki;
-    li[threadIdx.x] = i;
+#pragma unroll
+    for (int k = 0; k < D; ++k) x[k] = B.X[p][k][i];
   }
+  const uint64_t ki = x_order_key(x[0]);
+  lk[threadIdx.x] = ki;
+  li[threadIdx.x] = i;
   __syncthreads();
   if (!active) return;
-  const int p = B.ctl->parity;
-  double x[3];
-#pragma unroll
-  for (int k = 0; k < D; ++k) x[k] = B.X[p][k][i];
   const int32_t c = sd.key[i];
   const int32_t b = sd.start[c], e = sd.start[c + 1];
   const int32_t s0 = int32_t(a0), s1 = int32_t(a0 + kBlock < B.n ? a0 + kBlock : B.n);
@@ -1150,15 +1145,14 @@ __global__ __launch_bounds__(256) void kb_rank(SS ss) {
       for (int u = 0; u < 4; ++u) r += (k + u < e) & ((kv[u] < ki) | ((kv[u] == ki) & (v[u] < i)));
     }
   } else {
+    const double* X0 = B.X[p][0];
     for (int32_t k = b; k < e; k += 4) {
       int32_t v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = sd.tmp[k + u < e ? k + u : e - 1];
       uint64_t kv[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int32_t j = k + u < e ? k + u : e - 1;
-        v[u] = sd.tmp[j];
-        kv[u] = K[j];
-      }
+      for (int u = 0; u < 4; ++u) kv[u] = x_order_key(X0[v[u]]);
 #pragma unroll
       for (int u = 0; u < 4; ++u) r += (k + u < e) & ((kv[u] < ki) | ((kv[u] == ki) & (v[u] < i)));
     }
