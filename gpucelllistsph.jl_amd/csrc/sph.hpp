@@ -120,8 +120,10 @@ struct SphBufs {
   void* PRH;
 };
 
-// What a pair pass reads of a partner, packed into one 16-byte-aligned record
-// so that a partner costs a few 16-byte loads instead of one load per field.
+// What a pair pass reads of a partner, packed into one record of 16-byte words
+// so that a partner costs a few 16-byte loads instead of one load per field
+// (in 2D the arrays keep word i of every particle's record together:
+// RecordWords below).
 // PartnerN: the state at tⁿ read by pass 1 — written by the reorder, by pass 2
 // for the next step and by the corrections that change x, v or ρ.
 // PartnerH: the half-step state read by pass 2, written by pass 1.  Pass 2
@@ -206,6 +208,24 @@ struct alignas(16) PartnerH64 {
 };
 static_assert(sizeof(PartnerN64<2>) == 64 && sizeof(PartnerN64<3>) == 80, "partner record sizes");
 static_assert(sizeof(PartnerH64<2>) == 80 && sizeof(PartnerH64<3>) == 96, "partner record sizes");
+
+// How an array keeps its records (ld_rec / st_rec): in 2D as 16-byte words,
+// word i of every particle together (the lanes of a wave, whose partners lie
+// near each other in the x-ordered rows, then touch half the cache lines per
+// load), in 3D whole records in a row (a particle's ~160 partners in 27
+// cells are spread out, and a record split over two lines costs two).
+template <typename T>
+struct RecordWords : std::false_type {};
+template <>
+struct RecordWords<PartnerN<2>> : std::true_type {};
+template <>
+struct RecordWords<PartnerH<2>> : std::true_type {};
+template <>
+struct RecordWords<PartnerHX<2>> : std::true_type {};
+template <>
+struct RecordWords<PartnerN64<2>> : std::true_type {};
+template <>
+struct RecordWords<PartnerH64<2>> : std::true_type {};
 
 // Record types of a precision: N (state at tⁿ, pass 1 and the corrections) and
 // H (pass 2; FP32-split uses PartnerH or PartnerHX by step, see above).

@@ -384,32 +384,33 @@ __device__ __forceinline__ T ldo(const T* base, uint32_t byte_off) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
-// Record b of a partner-record array as 16-byte loads (64-bit offsets: the
-// records are larger than the 8 bytes the 32-bit offsets above are sized for).
+// Record b of a partner-record array (n particles) as 16-byte loads: word i at
+// word i·n + b where RecordWords<T> says the array keeps words apart (2D),
+// else at word b·(sizeof(T)/16) + i.  Offsets are 64-bit (the arrays are larger
+// than the 8-byte elements the 32-bit offsets above are sized for).
 template <typename T>
-__device__ __forceinline__ T ld_rec(const void* base, int64_t b) {
+__device__ __forceinline__ T ld_rec(const void* base, int64_t b, int64_t n) {
   static_assert(sizeof(T) % 16 == 0, "records are whole 16-byte words");
-  const uint4* p = reinterpret_cast<const uint4*>(static_cast<const char*>(base) + b * int64_t(sizeof(T)));
-  uint4 w[sizeof(T) / 16];
+  constexpr int kW = int(sizeof(T) / 16);
+  constexpr bool apart = RecordWords<T>::value;
+  const uint4* p = reinterpret_cast<const uint4*>(base) + (apart ? b : b * kW);
+  uint4 w[kW];
 #pragma unroll
-  for (int i = 0; i < int(sizeof(T) / 16); ++i) w[i] = p[i];
+  for (int i = 0; i < kW; ++i) w[i] = p[apart ? i * n : i];
   T r;
   __builtin_memcpy(&r, w, sizeof(T));
   return r;
 }
 
 template <typename T>
-__device__ __forceinline__ void st_rec(void* base, int64_t a, const T& r) {
-  uint4 w[sizeof(T) / 16];
+__device__ __forceinline__ void st_rec(void* base, int64_t a, const T& r, int64_t n) {
+  constexpr int kW = int(sizeof(T) / 16);
+  constexpr bool apart = RecordWords<T>::value;
+  uint4 w[kW];
   __builtin_memcpy(w, &r, sizeof(T));
-  uint4* p = reinterpret_cast<uint4*>(static_cast<char*>(base) + a * int64_t(sizeof(T)));
+  uint4* p = reinterpret_cast<uint4*>(base) + (apart ? a : a * kW);
 #pragma unroll
-  for (int i = 0; i < int(sizeof(T) / 16); ++i) p[i] = w[i];
-}
-
-template <int D, typename R>
-__device__ __forceinline__ typename Records<D, R>::N* rec_n(void* base) {
-  return static_cast<typename Records<D, R>::N*>(base);
+  for (int i = 0; i < kW; ++i) p[apart ? i * n : i] = w[i];
 }
 
 // ---- 3D cell frames and per-cell membership windows

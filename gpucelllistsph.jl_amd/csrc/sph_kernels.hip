@@ -381,7 +381,7 @@ __device__ __forceinline__ void reorder_one(const SphBufs<R>& B, const SphConst&
   } else {
     B.PN[s] = eos_f(rv, pt, C);
   }
-  st_rec(B.PRN, s, rec);
+  st_rec(B.PRN, s, rec, B.n);
   if (!F.synced_prev) B.RH[s] = B.RH2[src];
   B.PT[pq][s] = pt;
   B.ID[pq][s] = B.ID[pp][src];
@@ -515,7 +515,7 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
   }
   // partners from the records, which hold the updated state (pass 2)
   using Nb = typename Records<D, R>::N;
-  auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b); };
+  auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b, B.n); };
   const R Pa = B.PN[a];
   if (F.want_out) B.PBUF[a] = Pa;
   if (pta <= 0) {  // every DPC pair needs ptype > 0 on both sides
@@ -612,6 +612,8 @@ __global__ void k_dpc_apply(SS ss) {
   if (a >= B.n || B.PT[pp][a] <= 1) return;
   double fr[3];
   particle_frame<D, R>(B, C, G, a, fr);
+  using N = typename Records<D, R>::N;
+  N rec = ld_rec<N>(B.PRN, a, B.n);
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     R dv = B.DV[d][a];
@@ -619,9 +621,10 @@ __global__ void k_dpc_apply(SS ss) {
     const double nx = B.X[q][d][a] + double(dv) * dt;
     B.V[q][d][a] = nv;
     B.X[q][d][a] = nx;
-    rec_n<D, R>(B.PRN)[a].v[d] = nv;
-    put_x(rec_n<D, R>(B.PRN)[a], d, nx, fr);
+    rec.v[d] = nv;
+    put_x(rec, d, nx, fr);
   }
+  st_rec(B.PRN, a, rec, B.n);
 }
 
 // Partner records of the XSPH pass in the Records::N layout: xⁿ (the
@@ -644,7 +647,7 @@ __global__ void k_xsph_rec(SS ss) {
   }
   rec.rho = B.RHO[q][a];
   rec.pt = B.PT[pp][a];
-  st_rec(B.PRH, a, rec);
+  st_rec(B.PRH, a, rec, B.n);
 }
 
 // xsphcorr! + update_xsphcorr! (src/gpukernels_sph.jl:1757-1838): W at xⁿ.
@@ -675,7 +678,7 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
   }
   // xⁿ, v, ρ and ptype of a partner from the records of k_xsph_rec
   using Nb = typename Records<D, R>::N;
-  auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b); };
+  auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b, B.n); };
   double out[3];
   if constexpr (std::is_same<R, double>::value) {
     const double rhoa = RHO[a];
@@ -715,12 +718,15 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
   }
   double fr[3];
   particle_frame<D, R>(B, C, G, a, fr);
+  using N = typename Records<D, R>::N;
+  N rec = ld_rec<N>(B.PRN, a, B.n);
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     const double nx = B.X[q][d][a] + out[d] * dt;
     B.X[q][d][a] = nx;
-    put_x(rec_n<D, R>(B.PRN)[a], d, nx, fr);
+    put_x(rec, d, nx, fr);
   }
+  st_rec(B.PRN, a, rec, B.n);
 }
 
 // cspmcorr! (src/gpukernels_sph.jl:1693-1719): ρ = Σ m0 W / Σ m0 W / ρ_b, W at xⁿ.
@@ -803,13 +809,19 @@ __global__ void k_cspm_apply(SS ss) {
     B.RHO[q][a] = nr;
     const double P = eos_d(nr, pt, C);
     B.PN[a] = P;
-    rec_n<D, R>(B.PRN)[a].rho = nr;
-    rec_n<D, R>(B.PRN)[a].P = P;
+    using N = typename Records<D, R>::N;
+    N rec = ld_rec<N>(B.PRN, a, B.n);
+    rec.rho = nr;
+    rec.P = P;
+    st_rec(B.PRN, a, rec, B.n);
   } else {
     const float d = float(nr);
     B.RHO[q][a] = d;
     B.PN[a] = eos_f(d, pt, C);
-    rec_n<D, R>(B.PRN)[a].rho = d;
+    using N = typename Records<D, R>::N;
+    N rec = ld_rec<N>(B.PRN, a, B.n);
+    rec.rho = d;
+    st_rec(B.PRN, a, rec, B.n);
   }
 }
 

@@ -66,7 +66,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
     const double rhoa = sa;
     const double Pa = B.PN[a];
     using Nb = PartnerN64<D>;
-    auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b); };
+    auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b, B.n); };
     for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
       const int32_t ptb = q.pt;
       double dx[3], dv[3], gw[3];
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
     rec.rh = rv;
     rec.P = Ph;
     rec.pt = pta;
-    st_rec(B.PRH, a, rec);
+    st_rec(B.PRH, a, rec, B.n);
   } else {
     // ---------------- FP32-split
 #pragma clang fp contract(fast)
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
     const float mPa = -C.f_m0 * inv_rhoa;  // -m0/ρ_a of the pressure term
     const float Pa = B.PN[a];
     using Nb = PartnerN<D>;
-    auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b); };
+    auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRN, b, B.n); };
     const FrameK3 kt = particle_k<D, R>(B, C, G, a, xa);
     // ∇W = dx·dwr, so every term along ∇W is gathered into one coefficient fg
     // of dx, and dx·∇W = r²·dwr, dv·∇W = (dv·dx)·dwr.
@@ -293,8 +293,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
     }
     rec.h.rh = rv;
     rec.h.pt = pta;
-    if (F.synced_prev) st_rec(B.PRH, a, rec.h);  // pass 2 derives xⁿ
-    else st_rec(B.PRH, a, rec);
+    if (F.synced_prev) st_rec(B.PRH, a, rec.h, B.n);  // pass 2 derives xⁿ
+    else st_rec(B.PRH, a, rec, B.n);
   }
 }
 
@@ -334,7 +334,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       const double rhoa = sha;
       Pha = B.PH[a];
       using Nb = PartnerH64<D>;
-      auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b); };
+      auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b, B.n); };
       for_each_partner_fetch<Pipe, D>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q) {
         const int32_t ptb = q.pt;
         double dxn[3], dx[3], dv[3], gw[3];
@@ -442,7 +442,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       rec.rho = rv;
       rec.P = Pn;
       rec.pt = pta;
-      st_rec(B.PRN, a, rec);  // partner record of the next step's pass 1
+      st_rec(B.PRN, a, rec, B.n);  // partner record of the next step's pass 1
       if (F.ts) {
         red_acc = sqrt(dotd<D>(av, av));
         if (F.visc_in_pass2) red_visc = fabs(C.h * dotd<D>(vn, xn) / (dotd<D>(xn, xn) + C.dt_eta2));
@@ -461,11 +461,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       auto fetch = [&](const int32_t b) {
         Nb r;
         if (derive) {
-          r.h = ld_rec<PartnerH<D>>(B.PRH, b);
+          r.h = ld_rec<PartnerH<D>>(B.PRH, b, B.n);
 #pragma unroll
           for (int d = 0; d < D; ++d) r.dn[d] = 0.0f;
         } else {
-          r = ld_rec<Nb>(B.PRH, b);
+          r = ld_rec<Nb>(B.PRH, b, B.n);
         }
         return r;
       };
@@ -598,7 +598,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       }
       rec.rho = rv;
       rec.pt = pta;
-      st_rec(B.PRN, a, rec);  // partner record of the next step's pass 1
+      st_rec(B.PRN, a, rec, B.n);  // partner record of the next step's pass 1
       if (F.ts) {
         float a2 = dotf<D>(av, av);
         red_acc = sqrt(double(a2));
