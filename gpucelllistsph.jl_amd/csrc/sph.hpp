@@ -133,9 +133,10 @@ struct SphBufs {
 // rounding; otherwise the records are PartnerHX, which append x½ - xⁿ in FP32
 // (a displacement of order v·Δt, so its FP32 rounding lies far below the FP32
 // rounding of the separation itself; the step flag synced_prev tells both
-// passes which layout the step uses).  The
+// passes which layout the step uses).  In 3D the
 // pressure is not stored: it is the EOS of (rho, pt), recomputed with the same
-// function that fills PN / PH.
+// function that fills PN / PH; the 2D records store it (the EOS is ~10 VALU
+// of a pair's ~110, 2-3 % of a pass).
 //
 // In 3D the position is held as FP32 offsets from the origin of the
 // particle's cell in the current sort (its cell frame, cell_frame below): a
@@ -149,12 +150,16 @@ struct SphBufs {
 // (binned into an edge cell) sits further from its frame and loses
 // precision in proportion.  2D keeps FP64 positions: its record is two loads
 // either way.
+// The 2D records hold the partner's pressure (the EOS of (rho, pt)) in the
+// place of its type; the type's class (< 0, 0, 1, > 1: all that a pair term
+// asks of a partner's type) rides in the two lowest mantissa bits of the FP64
+// y (set_pt / rec_pt).
 template <int D>
 struct alignas(16) PartnerN {
   double x[D];
   float v[D];
   float rho;  // ρ - ρ0
-  int32_t pt;
+  float P;
 };
 template <>
 struct alignas(16) PartnerN<3> {
@@ -168,7 +173,7 @@ struct alignas(16) PartnerH {
   double xh[D];
   float vh[D];
   float rh;  // ρ½ - ρ0
-  int32_t pt;
+  float P;   // P½
 };
 template <>
 struct alignas(16) PartnerH<3> {

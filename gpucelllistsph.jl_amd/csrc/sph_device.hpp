@@ -504,6 +504,53 @@ __device__ __forceinline__ float sep_x(const double* xa, const PartnerH<D>& q, i
 __device__ __forceinline__ float sep_x(const double*, const PartnerH<3>& q, int d, const Win3& w) {
   return w.k[d] - q.xo[d];
 }
+// A partner's type as the pair kernels read it, and its pressure.  2D
+// records keep the pressure in the type's place and the type's class in the
+// two lowest mantissa bits of the FP64 y, read back as the representative
+// type -1, 0, 1 or 2 (the same answer to every test a pair term makes: > 0,
+// > 1, < 0, <= 1).  Those bits move y by at most 3 ulp of FP64, far below the
+// FP32 rounding of a separation; the membership store reads the position
+// arrays, not the records.
+__device__ __forceinline__ double with_class(double y, int32_t pt) {
+  const uint32_t c = pt < 0 ? 0u : (pt == 0 ? 1u : (pt == 1 ? 2u : 3u));
+  const uint64_t u = uint64_t(__double_as_longlong(y));
+  return __longlong_as_double((long long)((u & ~uint64_t(3)) | c));
+}
+__device__ __forceinline__ int32_t class_pt(double y) { return int32_t(uint32_t(__double_as_longlong(y)) & 3u) - 1; }
+template <int D>
+__device__ __forceinline__ int32_t rec_pt(const PartnerN<D>& r) {
+  return class_pt(r.x[1]);
+}
+__device__ __forceinline__ int32_t rec_pt(const PartnerN<3>& r) { return r.pt; }
+template <int D>
+__device__ __forceinline__ int32_t rec_pt(const PartnerH<D>& r) {
+  return class_pt(r.xh[1]);
+}
+__device__ __forceinline__ int32_t rec_pt(const PartnerH<3>& r) { return r.pt; }
+template <int D>
+__device__ __forceinline__ float rec_P(const PartnerN<D>& r, const SphConst&) {
+  return r.P;
+}
+__device__ __forceinline__ float rec_P(const PartnerN<3>& r, const SphConst& C) { return eos_f(r.rho, r.pt, C); }
+template <int D>
+__device__ __forceinline__ float rec_P(const PartnerH<D>& r, const SphConst&) {
+  return r.P;
+}
+__device__ __forceinline__ float rec_P(const PartnerH<3>& r, const SphConst& C) { return eos_f(r.rh, r.pt, C); }
+// Type and pressure into a record whose position is already in place.
+template <int D>
+__device__ __forceinline__ void set_pt(PartnerN<D>& r, int32_t pt, float P) {
+  r.x[1] = with_class(r.x[1], pt);
+  r.P = P;
+}
+__device__ __forceinline__ void set_pt(PartnerN<3>& r, int32_t pt, float) { r.pt = pt; }
+template <int D>
+__device__ __forceinline__ void set_pt(PartnerH<D>& r, int32_t pt, float P) {
+  r.xh[1] = with_class(r.xh[1], pt);
+  r.P = P;
+}
+__device__ __forceinline__ void set_pt(PartnerH<3>& r, int32_t pt, float) { r.pt = pt; }
+
 // Frame origins of particle a (3D; nothing to compute in 2D).
 template <int D, typename R>
 __device__ __forceinline__ void particle_frame(const SphBufs<R>& B, const SphConst& C, const CellGeom& G, int64_t a,

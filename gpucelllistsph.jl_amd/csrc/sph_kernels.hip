@@ -373,13 +373,15 @@ __device__ __forceinline__ void reorder_one(const SphBufs<R>& B, const SphConst&
     rec.v[k] = vs[k];
   }
   rec.rho = rv;
-  rec.pt = pt;
   if constexpr (std::is_same<R, double>::value) {
     const double P = eos_d(rv, pt, C);
     B.PN[s] = P;
     rec.P = P;
+    rec.pt = pt;
   } else {
-    B.PN[s] = eos_f(rv, pt, C);
+    const float P = eos_f(rv, pt, C);
+    B.PN[s] = P;
+    set_pt(rec, pt, P);
   }
   st_rec(B.PRN, s, rec, B.n);
   if (!F.synced_prev) B.RH[s] = B.RH2[src];
@@ -566,7 +568,7 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
     const FrameK3 kt = particle_k<D, R>(B, C, G, a, xa);
     for_each_partner_fetch<true, D, R, true>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n,
                                                                          const auto&... win) {
-      if (n.pt <= 0) return;
+      if (rec_pt(n) <= 0) return;
       float dx[3], dv[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -579,7 +581,7 @@ __global__ __launch_bounds__(256) void k_dpc_sum(SS ss) {
       const float w = W_f(u, C.f_dpc_wnorm);
       const float chi = __builtin_amdgcn_sqrtf(w * C.f_dpc_whinv);
       const float kk = u < C.f_dpc_lim ? 1.0f : chi;
-      const float lo = C.f_dpc_lambda * fabsf(Pa + eos_f(n.rho, n.pt, C)), cap = C.f_dpc_lambda * C.f_dpc_pmax;
+      const float lo = C.f_dpc_lambda * fabsf(Pa + rec_P(n, C)), cap = C.f_dpc_lambda * C.f_dpc_pmax;
       float pt = lo < cap ? lo : cap;
       pt = pt > C.f_dpc_pmin ? pt : C.f_dpc_pmin;
       const float Pb_dpc = chi * pt;
@@ -614,6 +616,7 @@ __global__ void k_dpc_apply(SS ss) {
   particle_frame<D, R>(B, C, G, a, fr);
   using N = typename Records<D, R>::N;
   N rec = ld_rec<N>(B.PRN, a, B.n);
+  const int32_t pt = B.PT[pp][a];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     R dv = B.DV[d][a];
@@ -624,6 +627,7 @@ __global__ void k_dpc_apply(SS ss) {
     rec.v[d] = nv;
     put_x(rec, d, nx, fr);
   }
+  if constexpr (!std::is_same<R, double>::value) set_pt(rec, pt, B.PN[a]);  // the 2D type class sits in y
   st_rec(B.PRN, a, rec, B.n);
 }
 
@@ -646,7 +650,8 @@ __global__ void k_xsph_rec(SS ss) {
     rec.v[d] = B.V[q][d][a];
   }
   rec.rho = B.RHO[q][a];
-  rec.pt = B.PT[pp][a];
+  if constexpr (std::is_same<R, double>::value) rec.pt = B.PT[pp][a];
+  else set_pt(rec, B.PT[pp][a], B.PN[a]);
   st_rec(B.PRH, a, rec, B.n);
 }
 
@@ -703,7 +708,7 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
     const FrameK3 kt = particle_k<D, R>(B, C, G, a, xa);
     for_each_partner_fetch<true, D, R, true>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& n,
                                                                          const auto&... win) {
-      if (n.pt <= 1) return;
+      if (rec_pt(n) <= 1) return;
       float dx[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) dx[d] = sep_x(xa, n, d, win...);
@@ -726,6 +731,7 @@ __global__ __launch_bounds__(256) void k_xsph(SS ss) {
     B.X[q][d][a] = nx;
     put_x(rec, d, nx, fr);
   }
+  if constexpr (!std::is_same<R, double>::value) set_pt(rec, pta, B.PN[a]);  // the 2D type class sits in y
   st_rec(B.PRN, a, rec, B.n);
 }
 
@@ -817,10 +823,12 @@ __global__ void k_cspm_apply(SS ss) {
   } else {
     const float d = float(nr);
     B.RHO[q][a] = d;
-    B.PN[a] = eos_f(d, pt, C);
+    const float P = eos_f(d, pt, C);
+    B.PN[a] = P;
     using N = typename Records<D, R>::N;
     N rec = ld_rec<N>(B.PRN, a, B.n);
     rec.rho = d;
+    set_pt(rec, pt, P);
     st_rec(B.PRN, a, rec, B.n);
   }
 }

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
     // ∇W = dx·dwr, so every term along ∇W is gathered into one coefficient fg
     // of dx, and dx·∇W = r²·dwr, dv·∇W = (dv·dx)·dwr.
     for_each_partner_fetch<Pipe, D, R, true>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& q, const auto&... win) {
-      const int32_t ptb = q.pt;
+      const int32_t ptb = rec_pt(q);
       float dx[3], dv[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
         const float sd = s - vd * (r2 * dwr) * C.f_m0 * inv_rhob;
         drho.add(both ? (pta > 1 ? sd : s) : 0.0f);
       }
-      const float Pt = (Pa + eos_f(drb, ptb, C)) * inv_rhob * mPa;
+      const float Pt = (Pa + rec_P(q, C)) * inv_rhob * mPa;
       float fg = same_kind(pta, ptb) ? Pt : 0.0f;
       {
         const float dmu = C.f_h * fminf(dvdx, 0.0f) * inv_den;  // AV acts on approaching pairs only
@@ -276,7 +276,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
     float rv = rh0 + drho.get() * hdt;
     if (rv < 0.0f && pta < 0) rv = 0.0f;
     B.RH[a] = rv;
-    B.PH[a] = eos_f(rv, pta, C);
+    const float Ph = eos_f(rv, pta, C);
+    B.PH[a] = Ph;
     PartnerHX<D> rec{};
     double fr[3];
     particle_frame<D, R>(B, C, G, a, fr);
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       rec.dn[d] = float(xh - xa[d]);
     }
     rec.h.rh = rv;
-    rec.h.pt = pta;
+    set_pt(rec.h, pta, Ph);
     if (F.synced_prev) st_rec(B.PRH, a, rec.h, B.n);  // pass 2 derives xⁿ
     else st_rec(B.PRH, a, rec, B.n);
   }
@@ -480,7 +481,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       for_each_partner_fetch<Pipe, D, R, true>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& r,
                                                                             const auto&... win) {
         const PartnerH<D>& q = r.h;
-        const int32_t ptb = q.pt;
+        const int32_t ptb = rec_pt(q);
         float dxn[3], dx[3], dv[3];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
             }
           }
         }
-        const float Pt = (Pha + eos_f(drb, ptb, C)) * inv_rhob * mPa;
+        const float Pt = (Pha + rec_P(q, C)) * inv_rhob * mPa;
         float fg = same_kind(pta, ptb) ? Pt : 0.0f;
         {
           const float dmu = C.f_h * fminf(dvdx, 0.0f) * inv_den;
@@ -566,7 +567,8 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       float rv = (2.0f * dr_n - epsi * (2.0f * C.f_rho0 + dr_n)) / (2.0f + epsi);
       if (rv < 0.0f && pta <= 1) rv = 0.0f;
       B.RHO[q][a] = rv;
-      B.PN[a] = eos_f(rv, pta, C);
+      const float Pn = eos_f(rv, pta, C);
+      B.PN[a] = Pn;
       if (F.corr_on) B.RH2[a] = rv;
       const float ml = pta > 1 ? 1.0f : 0.0f;
       double xn[3], vn[3];
@@ -597,7 +599,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
         }
       }
       rec.rho = rv;
-      rec.pt = pta;
+      set_pt(rec, pta, Pn);
       st_rec(B.PRN, a, rec, B.n);  // partner record of the next step's pass 1
       if (F.ts) {
         float a2 = dotf<D>(av, av);
