@@ -457,37 +457,33 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
       Pha = B.PH[a];
       // tⁿ separation from the half-step state when x½ = xⁿ + v½·Δt/2, else
       // from the displacement x½ - xⁿ that pass 1 appended to the records
-      const bool derive = F.synced_prev;
-      using Nb = PartnerHX<D>;
-      auto fetch = [&](const int32_t b) {
-        Nb r;
-        if (derive) {
-          r.h = ld_rec<PartnerH<D>>(B.PRH, b, B.n);
-#pragma unroll
-          for (int d = 0; d < D; ++d) r.dn[d] = 0.0f;
-        } else {
-          r = ld_rec<Nb>(B.PRH, b, B.n);
-        }
-        return r;
-      };
       // xⁿa - xⁿb = (x½a - x½b) - (da - db) with d = x½ - xⁿ, the displacement
-      // part in FP32 (its size is of order |v|·Δt): derived, d = v½·Δt/2; else
-      // d from the records
+      // part in FP32 (its size is of order |v|·Δt): derived, d = v½·Δt/2
+      // (PartnerH records); else d from the records (PartnerHX).  The choice
+      // holds for the whole step, so the loop is compiled once for each.
       const float fhdt = float(dt * 0.5);
       float dna[3];
 #pragma unroll
       for (int d = 0; d < D; ++d) dna[d] = float(xha[d] - xa[d]);
       const FrameK3 kt = particle_k<D, R>(B, C, G, a, xha);
+      auto gather = [&](auto derive_c) {
+      constexpr bool derive = decltype(derive_c)::value;
+      using Nb = std::conditional_t<derive, PartnerH<D>, PartnerHX<D>>;
+      auto fetch = [&](const int32_t b) { return ld_rec<Nb>(B.PRH, b, B.n); };
       for_each_partner_fetch<Pipe, D, R, true>(B, a, nc, G, XM, C.dist2, fetch, [&](const int32_t, const Nb& r,
                                                                             const auto&... win) {
-        const PartnerH<D>& q = r.h;
+        const PartnerH<D>* qp;
+        if constexpr (derive) qp = &r;
+        else qp = &r.h;
+        const PartnerH<D>& q = *qp;
         const int32_t ptb = rec_pt(q);
         float dxn[3], dx[3], dv[3];
 #pragma unroll
         for (int d = 0; d < D; ++d) {
           dx[d] = sep_x(xha, q, d, win...);
           dv[d] = vha[d] - q.vh[d];
-          dxn[d] = derive ? dx[d] - dv[d] * fhdt : dx[d] - (dna[d] - r.dn[d]);
+          if constexpr (derive) dxn[d] = dx[d] - dv[d] * fhdt;
+          else dxn[d] = dx[d] - (dna[d] - r.dn[d]);
         }
         // ∇W from pass 1 (positions xⁿ): ∇W = dxn·dwr; terms along ∇W are
         // gathered into one coefficient fg of dxn
@@ -556,6 +552,9 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(W))) void k_
 #pragma unroll
         for (int d = 0; d < D; ++d) acc[d].add(ax[d]);
       }, &kt);
+      };
+      if (F.synced_prev) gather(std::true_type{});
+      else gather(std::false_type{});
 #pragma unroll
       for (int d = 0; d < D; ++d) av[d] = acc[d].get();
       av[1] = av[1] - C.f_g;
