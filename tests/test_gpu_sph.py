@@ -381,8 +381,8 @@ def test_graph_replay_equals_eager_launches(G, monkeypatch, case):
 def test_config2_3d_f32split_vs_f64_500_steps(G, S):
     """The 3D dam break with viscosity, surface tension, DPC, XSPH and CSPM
     (883,376 particles at dx = 0.01) after 500 adaptive steps: FP32-split
-    against FP64, per field (measured: positions 5.6e-6·dx, density 8.7e-8
-    relative, velocity 5.0e-6 of the largest speed)."""
+    against FP64, per field (measured: positions 5.6e-6·dx, density 8.8e-8
+    relative, velocity 5.2e-6 of the largest speed)."""
     sc = S.config2(dx=0.01)
 
     def run(prec):
