@@ -1115,11 +1115,10 @@ __global__ void kb_scatter(SS ss) {
 // so they fill fewer stored windows, and in 3D the membership store bounds
 // each particle's candidates in a row to |x_a - x_b| < dist by bisection.
 // The order is the same for every run (ties in x by input index).  The
-// particle's
-// position moves to its new place here (loaded with the key, so it arrives
-// while the rank is counted), so the membership store after it reads the
-// candidates of a stencil row as contiguous ranges of the new position arrays
-// instead of through the permutation.
+// particle's position (loaded first: its x is the key) moves to its new place
+// here, so the membership store after it reads the candidates of a stencil
+// row as contiguous ranges of the new position arrays instead of through the
+// permutation.
 template <int D, typename R, typename SS>
 __global__ __launch_bounds__(256) void kb_rank(SS ss) {
   GCLSPH_SCENE(ss);
