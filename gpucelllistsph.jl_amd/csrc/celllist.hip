@@ -60,20 +60,28 @@ __global__ void k_cellmap(XPtrs xp, XPtrs xalt, const int* sel, int64_t n, CellG
   for (int k = 0; k < D; ++k) c[k] = cell_coord<T>(static_cast<const T*>(src.p[k])[i], g, k, use_off64);
   int64_t lin = int64_t(c[0] - 1) + g.grid[0] * (int64_t(c[1] - 1) + g.grid[1] * int64_t(c[2] - 1));
   const int32_t k = int32_t(lin);
-  key[i] = k;
+  // cell and arrival slot in visiting order t (contiguous stores)
+  key[t] = k;
   if (!aggregate) {
-    slot[i] = atomicAdd(&cnt[lin], 1);
+    slot[t] = atomicAdd(&cnt[lin], 1);
     return;
   }
-  slot[i] = wave_aggregated_slot(cnt, k);
+  slot[t] = wave_aggregated_slot(cnt, k);
 }
 
+// Item visited t-th (order as in k_cellmap) to its arrival slot: tmp holds
+// the item, skey its cell (every slot of a cell's range holds that cell, so
+// the rank below need not write it).
 __global__ void k_scatter(const int32_t* __restrict__ key, const int32_t* __restrict__ slot,
-                          const int32_t* __restrict__ start, int32_t* __restrict__ tmp, int64_t n, const int* flag) {
+                          const int32_t* __restrict__ start, int32_t* __restrict__ tmp, int32_t* __restrict__ skey,
+                          const int32_t* __restrict__ order, int64_t n, const int* flag) {
   if (flag && *flag == 0) return;
-  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  tmp[start[key[i]] + slot[i]] = int32_t(i);
+  int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t c = key[t];
+  const int32_t s = start[c] + slot[t];
+  tmp[s] = order ? order[t] : int32_t(t);
+  skey[s] = c;
 }
 
 // Rank each item inside its cell by input index: final order is (cell, input).
@@ -81,9 +89,9 @@ __global__ void k_scatter(const int32_t* __restrict__ key, const int32_t* __rest
 // the key, so it arrives while the rank is counted), which replaces a
 // separate gather of the positions.
 template <typename T, int D, bool Pos>
-__global__ void k_rank(const int32_t* __restrict__ key, const int32_t* __restrict__ start,
-                       const int32_t* __restrict__ tmp, int32_t* __restrict__ perm, int32_t* __restrict__ skey,
-                       int64_t n, const int* flag, XPtrs xp, XPtrs xalt, const int* sel, XPtrs xsp) {
+__global__ void k_rank(const int32_t* __restrict__ start, const int32_t* __restrict__ tmp, int32_t* __restrict__ perm,
+                       const int32_t* __restrict__ skey, int64_t n, const int* flag, XPtrs xp, XPtrs xalt,
+                       const int* sel, XPtrs xsp) {
   if (flag && *flag == 0) return;
   int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -94,7 +102,7 @@ __global__ void k_rank(const int32_t* __restrict__ key, const int32_t* __restric
 #pragma unroll
     for (int k = 0; k < D; ++k) x[k] = static_cast<const T*>(src.p[k])[i];
   }
-  int32_t c = key[i];
+  const int32_t c = skey[s];  // the scatter's, the same for every slot of the cell
   int32_t b = start[c], e = start[c + 1];
   // four loads in flight per trip (indices clamped into the cell, which holds
   // at least this item; the clamped ones are not counted)
@@ -107,7 +115,6 @@ __global__ void k_rank(const int32_t* __restrict__ key, const int32_t* __restric
     for (int u = 0; u < 4; ++u) r += (k + u < e) & (v[u] < i);
   }
   perm[b + r] = i;
-  skey[b + r] = c;
   if constexpr (Pos) {
 #pragma unroll
     for (int k = 0; k < D; ++k) static_cast<T*>(const_cast<void*>(xsp.p[k]))[b + r] = x[k];
@@ -451,17 +458,18 @@ void bin_sort_t(const CellGeom& g, const void* const* x, int64_t n, bool use_off
   GCL_HIP(hipGetLastError());
   fill_i32(sb.maxcell, 1, 0, flag, s);
   exclusive_scan_i32(sb.cnt, sb.start, nc, sb.scan_tmp, flag, s, sb.maxcell);
-  k_scatter<<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.slot, sb.start, sb.tmp, n, flag);
+  k_scatter<<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.slot, sb.start, sb.tmp, sb.skey, sb.sorted ? sb.perm : nullptr, n,
+                                           flag);
   GCL_HIP(hipGetLastError());
   const XPtrs xsp = xs ? XPtrs{{xs[0], xs[1], g.dim == 3 ? xs[2] : xs[1]}} : XPtrs{{nullptr, nullptr, nullptr}};
   if (!xs)
-    k_rank<T, 2, false><<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.start, sb.tmp, sb.perm, sb.skey, n, flag, xp, xa, sel,
+    k_rank<T, 2, false><<<grid_for(n), kBlock, 0, s>>>(sb.start, sb.tmp, sb.perm, sb.skey, n, flag, xp, xa, sel,
                                                         xsp);
   else if (g.dim == 2)
-    k_rank<T, 2, true><<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.start, sb.tmp, sb.perm, sb.skey, n, flag, xp, xa, sel,
+    k_rank<T, 2, true><<<grid_for(n), kBlock, 0, s>>>(sb.start, sb.tmp, sb.perm, sb.skey, n, flag, xp, xa, sel,
                                                        xsp);
   else
-    k_rank<T, 3, true><<<grid_for(n), kBlock, 0, s>>>(sb.key, sb.start, sb.tmp, sb.perm, sb.skey, n, flag, xp, xa, sel,
+    k_rank<T, 3, true><<<grid_for(n), kBlock, 0, s>>>(sb.start, sb.tmp, sb.perm, sb.skey, n, flag, xp, xa, sel,
                                                        xsp);
   GCL_HIP(hipGetLastError());
   if (!flag) sb.sorted = true;  // perm now holds a permutation of [0, n) (a flagged sort may not have run)
