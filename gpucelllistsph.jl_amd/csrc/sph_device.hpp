@@ -387,13 +387,14 @@ __device__ __forceinline__ T ldo(const T* base, uint32_t byte_off) {
 // Record b of a partner-record array (n particles) as 16-byte loads: word i at
 // word i·n + b where RecordWords<T> says the array keeps words apart (2D),
 // else at word b·(sizeof(T)/16) + i.  Offsets are 64-bit (the arrays are larger
-// than the 8-byte elements the 32-bit offsets above are sized for).
+// than the 8-byte elements the 32-bit offsets above are sized for); b, a
+// partner index in [0, n), is taken unsigned (no sign extension per load).
 template <typename T>
-__device__ __forceinline__ T ld_rec(const void* base, int64_t b, int64_t n) {
+__device__ __forceinline__ T ld_rec(const void* base, uint32_t b, int64_t n) {
   static_assert(sizeof(T) % 16 == 0, "records are whole 16-byte words");
   constexpr int kW = int(sizeof(T) / 16);
   constexpr bool apart = RecordWords<T>::value;
-  const uint4* p = reinterpret_cast<const uint4*>(base) + (apart ? b : b * kW);
+  const uint4* p = reinterpret_cast<const uint4*>(base) + (apart ? uint64_t(b) : uint64_t(b) * kW);
   uint4 w[kW];
 #pragma unroll
   for (int i = 0; i < kW; ++i) w[i] = p[apart ? i * n : i];
