@@ -191,6 +191,32 @@ def test_stored_membership_equals_stencil_scan(G, monkeypatch, prec, case):
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32split"])
+@pytest.mark.parametrize("case", ["dam2d", "dam2d_allphys", "dam3d_allphys"])
+def test_ptype_values_act_by_class(G, prec, case):
+    """The pair terms ask of a particle type only whether it is > 0, > 1, < 0 or
+    <= 1 (the ptype tests of src/gpukernels_sph.jl), so types moved inside
+    their class (2 -> 7, -1 -> -4; 0 and 1 are classes of their own) must give
+    bit-identical steps.  The 2D FP32-split partner records carry only the
+    class of a partner's type, read back as -1, 0, 1 or 2; this holds them to
+    it, with fluid (2), wall (0, 1) and ptype -1 particles in the scenes."""
+    sc, steps, ts = golden_cases()[case]
+
+    def run(pt):
+        cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, pt, v=sc.v, precision=prec)
+        upd, mx = G.stepsolve_(prob, steps + 3, timestepping=ts)
+        return [upd, mx] + [np.asarray(gpu_fields(G, prob)[f]) for f in FIELDS]
+
+    pt = np.asarray(sc.ptype)
+    moved = np.where(pt > 1, pt + 5, np.where(pt < 0, pt - 3, pt)).astype(pt.dtype)
+    assert (moved != pt).any()
+    ref = run(pt)
+    got = run(moved)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
 def test_user_update_and_set_points_between_calls(G, O, S):
     """update!(prob.system), partialupdate! and edits of prob.system.points between stepsolve! calls."""
     sc = golden_cases()["dam2d_allphys"][0]
