@@ -192,6 +192,42 @@ def test_stored_membership_equals_stencil_scan(G, monkeypatch, prec, case):
 
 
 @pytest.mark.parametrize("prec", ["f64", "f32split"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_membership_with_candidates_exactly_at_dist(G, S, monkeypatch, prec, dim):
+    """A lattice of spacing 0.25 with dist = cell size = 0.75 (all exact in
+    binary) puts many candidates at exactly |x_a - x_b| = dist, where the test
+    r² < dist² fails by the smallest margin: the membership store (in 3D
+    bounded to |x_a - x_b| < dist by bisection over the x-ordered rows) and the
+    full stencil scan (GCLSPH_NMASK_CAP=0) must give bit-identical steps."""
+    import dataclasses
+
+    if dim == 2:
+        sc = S.dam_break_2d(0.25, 1.5, 1.5, 3.0, 2.5)
+    else:
+        sc = S.dam_break_3d(0.25, (1.5, 1.5, 1.0), (3.0, 2.5, 2.0))
+    sc = dataclasses.replace(sc, dist=0.75, cellsize=(0.75,) * dim,
+                             params=dataclasses.replace(sc.params, nui=0.0))
+    x = np.asarray(sc.points[0])
+    ties = np.count_nonzero(np.abs(x[:, None] - x[None, :]) == 0.75)
+    assert ties > 1000  # the boundary case is exercised
+
+    def run(cap):
+        if cap is None:
+            monkeypatch.delenv("GCLSPH_NMASK_CAP", raising=False)
+        else:
+            monkeypatch.setenv("GCLSPH_NMASK_CAP", str(cap))
+        cl = G.GPUCellList(sc.points, sc.cellsize, sc.dist)
+        prob = G.SPHProblem.from_params(cl, sc.params, sc.rho, sc.ptype, precision=prec)
+        upd, mx = G.stepsolve_(prob, 4, timestepping=True)
+        return [upd, mx] + [np.asarray(gpu_fields(G, prob)[f]) for f in FIELDS]
+
+    ref = run(None)
+    got = run(0)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32split"])
 @pytest.mark.parametrize("case", ["dam2d", "dam2d_allphys", "dam3d_allphys"])
 def test_ptype_values_act_by_class(G, prec, case):
     """The pair terms ask of a particle type only whether it is > 0, > 1, < 0 or
